@@ -1,0 +1,152 @@
+/*
+ * pyiid_hip.h - C ABI of the MI355X (gfx950) backend for pyIID's scattering
+ * path: Debye F(Q), its position gradient, the F(Q) -> G(r) transform and the
+ * Rw / chi^2 energy and forces.
+ *
+ * Conventions
+ *  - Every call returns 0 on success and a negative PYIID_E* code on failure;
+ *    pyiid_last_error() then returns a message (thread-local).
+ *  - No exceptions cross the boundary.  Plain pointers and sizes only.
+ *  - Pointer arguments are host pointers unless PYIID_DEVICE is set in
+ *    `flags`, in which case they are device pointers on the context's device
+ *    (e.g. torch.Tensor.data_ptr()).  Host results are complete when the call
+ *    returns; device results are ordered on the context stream.
+ *  - A context owns one HIP stream (or borrows one, pyiid_set_stream), the
+ *    device-resident structure and tables, and its workspaces.  Calls on one
+ *    context must not overlap; distinct contexts are independent.
+ *
+ * Reference citations are path:line in the pyIID repository.
+ */
+#ifndef PYIID_HIP_H
+#define PYIID_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYIID_ABI_VERSION 1
+
+/* flags */
+#define PYIID_DEVICE 1 /* pointers are device pointers */
+#define PYIID_F64 2    /* gradient arrays are double (else float) */
+
+/* status codes */
+#define PYIID_OK 0
+#define PYIID_E_INVALID -1  /* bad argument / shape */
+#define PYIID_E_STATE -2    /* required state not set (positions, tables, transform) */
+#define PYIID_E_HIP -3      /* HIP runtime error */
+#define PYIID_E_NOMEM -4    /* device allocation failed */
+#define PYIID_E_NODEVICE -5 /* no HIP device */
+
+/* scatter-table slots: the two per-atom tables of _wrap_atoms
+ * (pyiid/experiments/elasticscatter/__init__.py:127-148) */
+#define PYIID_SLOT_FQ 0  /* 'F(Q) scatter' */
+#define PYIID_SLOT_PDF 1 /* 'PDF scatter' */
+
+/* potentials of Calc1D (pyiid/calc/calc_1d.py:28-34) */
+#define PYIID_POT_RW 0
+#define PYIID_POT_CHI_SQ 1
+
+typedef struct pyiid_ctx pyiid_ctx;
+
+/* ---- library / context ------------------------------------------------ */
+int pyiid_abi_version(void);
+const char *pyiid_last_error(void);
+/* Replaces the device probe check_gpu() / get_gpus_mem()
+ * (elasticscatter/__init__.py:29-38, experiments/__init__.py:8-19). */
+int pyiid_device_count(int *count);
+int pyiid_create(int device, pyiid_ctx **ctx);
+int pyiid_destroy(pyiid_ctx *ctx);
+/* Use `stream` (a hipStream_t; NULL restores the context's own stream). */
+int pyiid_set_stream(pyiid_ctx *ctx, void *stream);
+int pyiid_synchronize(pyiid_ctx *ctx);
+/* Kernel timing with HIP events on the launch stream: when enabled, every
+ * launch of the Debye kernels is bracketed by events and its duration summed.
+ * kind: 0 = fused F+grad kernel, 1 = F-only kernel, 2 = transform GEMM,
+ *       3 = pair-geometry kernel. */
+int pyiid_enable_timing(pyiid_ctx *ctx, int enable);
+int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launches);
+
+/* ---- structure and tables ---------------------------------------------- */
+/* Atom positions, float32 N x 3 (the reference casts positions to float32:
+ * flat_multi_cpu_wrap.py:9-19, gpu_wrap.py:12-24). */
+int pyiid_set_positions(pyiid_ctx *ctx, const float *pos, int64_t n, int flags);
+/* Element index per atom in [0, n_elem) (int32, N).  Rows of the scatter
+ * tables below are indexed by it.  Replaces the per-atom N x Q float arrays
+ * 'F(Q) scatter' / 'PDF scatter' that _wrap_atoms stores on the atoms
+ * (elasticscatter/__init__.py:127-148) by an E x Q table plus this index. */
+int pyiid_set_elements(pyiid_ctx *ctx, const int32_t *elem, int64_t n, int flags);
+/* Per-element form-factor table f[e][q], float32 n_elem x n_q, for `slot`. */
+int pyiid_set_scatter(pyiid_ctx *ctx, int slot, const float *table, int32_t n_elem,
+                      int32_t n_q, int flags);
+
+/* ---- Debye sums -------------------------------------------------------- */
+/* F(Q) on the full grid from Q=0: 2/na * sum_{i>j} f_i f_j sin(Q r)/r with
+ * Q = m * float32(qbin), nan_to_num.  Writes n_q floats.
+ * Replaces wrap_fq(atoms, qbin, sum_type) of the CPU and GPU processors
+ * (cpu_wrappers/flat_multi_cpu_wrap.py:22-60, gpu_wrappers/gpu_wrap.py:119-156). */
+int pyiid_fq(pyiid_ctx *ctx, int slot, double qbin, float *fq, int flags);
+/* -1/2 dF/dq as the reference returns it: (N, 3, n_q), Q fastest; float32,
+ * or double with PYIID_F64.  `fq` (nullable) receives F(Q) from the same
+ * fused kernel.  Replaces wrap_fq_grad(atoms, qbin, sum_type)
+ * (flat_multi_cpu_wrap.py:63-102, gpu_wrap.py:159-194). */
+int pyiid_grad_fq(pyiid_ctx *ctx, int slot, double qbin, void *grad, float *fq, int flags);
+/* Flat pair geometry for pairs k in [k_offset, k_offset + k_count) with
+ * k = j + i(i-1)/2, i > j: d[k] = q_i - q_j (k x 3 float) and r[k] = |d[k]|.
+ * Replaces get_d_array + get_r_array (kernels/cpu_flat.py:14-52,
+ * kernels/gpu_flat.py:11-49).  d or r may be NULL. */
+int pyiid_pair_geometry(pyiid_ctx *ctx, int64_t k_offset, int64_t k_count, float *d, float *r,
+                        int flags);
+/* na(Q) = N * mean_{i>j} f_i f_j for `slot` (double, n_q).  Replaces
+ * get_normalization_array + the mean in the wrappers
+ * (cpu_flat.py:55-74, flat_multi_cpu_wrap.py:52-54). */
+int pyiid_normalization(pyiid_ctx *ctx, int slot, double *na, int flags);
+
+/* ---- F(Q) -> G(r) ------------------------------------------------------ */
+/* Build the transform of get_pdf_at_qmin (kernels/master_kernel.py:39-104)
+ * for F of length n_q on the grid rgrid (n_r doubles, host memory). */
+int pyiid_set_transform(pyiid_ctx *ctx, double rstep, double qstep, const double *rgrid,
+                        int32_t n_r, double qmin, int32_t n_q);
+/* G(r) from F(Q) (n_q floats) -> n_r doubles.  Replaces get_pdf_at_qmin. */
+int pyiid_pdf(pyiid_ctx *ctx, const float *fq, double *gr, int flags);
+/* grad G = grad F . T^T for `rows` rows of length n_q (float, or double with
+ * PYIID_F64) -> rows x n_r doubles.  Replaces grad_pdf
+ * (master_kernel.py:272-290, gpu_wrap.py:197-284); rows = 3N. */
+int pyiid_grad_pdf(pyiid_ctx *ctx, const void *grad_fq, int64_t rows, double *grad_gr,
+                   int flags);
+
+/* ---- Rw / chi^2 -------------------------------------------------------- */
+/* Rw (or chi^2) and scale of gcalc against gobs (n doubles each).
+ * Replaces get_rw / get_chi_sq (master_kernel.py:206-266). */
+int pyiid_potential(pyiid_ctx *ctx, int potential, const double *gobs, const double *gcalc,
+                    int32_t n, double *value, double *scale, int flags);
+/* Gradient of the potential for `rows` rows of grad_gcalc (rows x n doubles)
+ * -> rows doubles.  Replaces get_grad_rw / get_grad_chi_sq
+ * (master_kernel.py:293-375) as called by wrap_grad_rw / wrap_grad_chi_sq
+ * (calc/__init__.py:56-105). */
+int pyiid_potential_grad(pyiid_ctx *ctx, int potential, const double *grad_gcalc, int64_t rows,
+                         const double *gobs, const double *gcalc, int32_t n, double *grad,
+                         int flags);
+/* Calc1D energy and forces for a PDF target (calc_1d.py:78-95 with
+ * exp_function=get_pdf, exp_grad_function=get_grad_pdf):
+ * energy = conv * P(G), forces = conv * dP/dG . gradG in the reference's
+ * convention (gradG built from the -1/2 dF/dq array).  G is computed from
+ * the `slot` table at `qbin`, gobs has the transform's n_r points.  The
+ * gradient array is never materialised: the adjoint u = T^T w is contracted
+ * inside the Debye gradient kernel.  forces: N x 3 doubles (nullable). */
+int pyiid_pdf_energy_forces(pyiid_ctx *ctx, int slot, double qbin, const double *gobs,
+                            int potential, double conv, double *energy, double *forces,
+                            int flags);
+/* Same for an F(Q) target (exp_function=get_fq, exp_grad_function=get_grad_fq):
+ * gobs has n_q - q_first points, the F grid sliced from bin q_first
+ * (elasticscatter/__init__.py:331-332,495-496). */
+int pyiid_fq_energy_forces(pyiid_ctx *ctx, int slot, double qbin, int32_t q_first,
+                           const double *gobs, int potential, double conv, double *energy,
+                           double *forces, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYIID_HIP_H */

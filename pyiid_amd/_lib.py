@@ -1,0 +1,266 @@
+"""ctypes binding of ``libpyiid_hip.so`` (C ABI declared in include/pyiid_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C
+pyiid_amd/csrc``) into ``pyiid_amd/_lib/libpyiid_hip.so``.  There is no
+fallback: if the library or a HIP device is missing, every entry point raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libpyiid_hip.so")
+
+PYIID_DEVICE = 1
+PYIID_F64 = 2
+SLOT_FQ = 0
+SLOT_PDF = 1
+POT_RW = 0
+POT_CHI_SQ = 1
+
+KERNEL_FUSED = 0
+KERNEL_FQ = 1
+KERNEL_GEMM = 2
+KERNEL_PAIRS = 3
+
+_ERRORS = {
+    -1: ValueError,
+    -2: RuntimeError,
+    -3: RuntimeError,
+    -4: MemoryError,
+    -5: RuntimeError,
+}
+
+# (name, argtypes) for every entry point of include/pyiid_hip.h
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_SIGNATURES = [
+    ("pyiid_abi_version", []),
+    ("pyiid_device_count", [ctypes.POINTER(_I)]),
+    ("pyiid_create", [_I, ctypes.POINTER(_P)]),
+    ("pyiid_destroy", [_P]),
+    ("pyiid_set_stream", [_P, _P]),
+    ("pyiid_synchronize", [_P]),
+    ("pyiid_enable_timing", [_P, _I]),
+    ("pyiid_kernel_time", [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    ("pyiid_set_positions", [_P, _P, _I64, _I]),
+    ("pyiid_set_elements", [_P, _P, _I64, _I]),
+    ("pyiid_set_scatter", [_P, _I, _P, _I32, _I32, _I]),
+    ("pyiid_fq", [_P, _I, _D, _P, _I]),
+    ("pyiid_grad_fq", [_P, _I, _D, _P, _P, _I]),
+    ("pyiid_pair_geometry", [_P, _I64, _I64, _P, _P, _I]),
+    ("pyiid_normalization", [_P, _I, _P, _I]),
+    ("pyiid_set_transform", [_P, _D, _D, _P, _I32, _D, _I32]),
+    ("pyiid_pdf", [_P, _P, _P, _I]),
+    ("pyiid_grad_pdf", [_P, _P, _I64, _P, _I]),
+    ("pyiid_potential", [_P, _I, _P, _P, _I32, ctypes.POINTER(_D), ctypes.POINTER(_D), _I]),
+    ("pyiid_potential_grad", [_P, _I, _P, _I64, _P, _P, _I32, _P, _I]),
+    ("pyiid_pdf_energy_forces", [_P, _I, _D, _P, _I, _D, ctypes.POINTER(_D), _P, _I]),
+    ("pyiid_fq_energy_forces", [_P, _I, _D, _I32, _P, _I, _D, ctypes.POINTER(_D), _P, _I]),
+]
+EXPORTED = [name for name, _ in _SIGNATURES] + ["pyiid_last_error"]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load(path=LIB_PATH):
+    """Load the shared library (once) and declare every entry point."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise RuntimeError(
+                "libpyiid_hip.so is not built (%s); run __graft_entry__.build()" % path)
+        lib = ctypes.CDLL(path)
+        for name, argtypes in _SIGNATURES:
+            fn = getattr(lib, name)
+            fn.argtypes = argtypes
+            fn.restype = ctypes.c_int
+        lib.pyiid_last_error.argtypes = []
+        lib.pyiid_last_error.restype = ctypes.c_char_p
+        _lib = lib
+        return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().pyiid_last_error().decode(errors="replace")
+        raise _ERRORS.get(rc, RuntimeError)("pyiid_hip error %d: %s" % (rc, msg))
+    return rc
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(load().pyiid_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _ptr(a):
+    """Host pointer of a contiguous numpy array, or a device pointer (int)."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    return a.data_ptr()  # torch tensor
+
+
+class Context:
+    """One device context: a HIP stream plus the device-resident structure,
+    scatter tables, transform and workspaces (``pyiid_ctx``)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        check(self.lib.pyiid_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.pyiid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- plumbing ---------------------------------------------------------
+    def set_stream(self, stream_handle):
+        check(self.lib.pyiid_set_stream(self.handle, stream_handle))
+
+    def synchronize(self):
+        check(self.lib.pyiid_synchronize(self.handle))
+
+    def enable_timing(self, on=True):
+        check(self.lib.pyiid_enable_timing(self.handle, 1 if on else 0))
+
+    def kernel_time(self, kind):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(self.lib.pyiid_kernel_time(self.handle, kind, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    # --- structure ---------------------------------------------------------
+    def set_positions(self, pos, flags=0):
+        if flags & PYIID_DEVICE:
+            n = pos.shape[0]
+        else:
+            pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+            n = pos.shape[0]
+        check(self.lib.pyiid_set_positions(self.handle, _ptr(pos), n, flags))
+
+    def set_elements(self, elem, flags=0):
+        if not flags & PYIID_DEVICE:
+            elem = np.ascontiguousarray(elem, dtype=np.int32)
+        check(self.lib.pyiid_set_elements(self.handle, _ptr(elem), elem.shape[0], flags))
+
+    def set_scatter(self, slot, table, flags=0):
+        if not flags & PYIID_DEVICE:
+            table = np.ascontiguousarray(table, dtype=np.float32)
+        e, q = table.shape
+        check(self.lib.pyiid_set_scatter(self.handle, slot, _ptr(table), e, q, flags))
+
+    # --- Debye -------------------------------------------------------------
+    def fq(self, slot, qbin, nq, out=None, flags=0):
+        if out is None:
+            out = np.empty(nq, np.float32)
+        check(self.lib.pyiid_fq(self.handle, slot, float(qbin), _ptr(out), flags))
+        return out
+
+    def grad_fq(self, slot, qbin, n, nq, out=None, fq_out=None, f64=False, flags=0):
+        if f64:
+            flags |= PYIID_F64
+        if out is None:
+            out = np.empty((n, 3, nq), np.float64 if f64 else np.float32)
+        check(self.lib.pyiid_grad_fq(self.handle, slot, float(qbin), _ptr(out), _ptr(fq_out),
+                                     flags))
+        return out
+
+    def pair_geometry(self, k_offset, k_count, want_d=True, want_r=True):
+        d = np.empty((k_count, 3), np.float32) if want_d else None
+        r = np.empty(k_count, np.float32) if want_r else None
+        check(self.lib.pyiid_pair_geometry(self.handle, int(k_offset), int(k_count), _ptr(d),
+                                           _ptr(r), 0))
+        return d, r
+
+    def normalization(self, slot, nq):
+        out = np.empty(nq, np.float64)
+        check(self.lib.pyiid_normalization(self.handle, slot, _ptr(out), 0))
+        return out
+
+    # --- transform ---------------------------------------------------------
+    def set_transform(self, rstep, qstep, rgrid, qmin, nq):
+        rgrid = np.ascontiguousarray(rgrid, dtype=np.float64)
+        check(self.lib.pyiid_set_transform(self.handle, float(rstep), float(qstep), _ptr(rgrid),
+                                           len(rgrid), float(qmin), int(nq)))
+
+    def pdf(self, fq, n_r, out=None, flags=0):
+        if not flags & PYIID_DEVICE:
+            fq = np.ascontiguousarray(fq, dtype=np.float32)
+        if out is None:
+            out = np.empty(n_r, np.float64)
+        check(self.lib.pyiid_pdf(self.handle, _ptr(fq), _ptr(out), flags))
+        return out
+
+    def grad_pdf(self, grad_fq, n_r, out=None, flags=0):
+        if not flags & PYIID_DEVICE:
+            if grad_fq.dtype == np.float64:
+                flags |= PYIID_F64
+                grad_fq = np.ascontiguousarray(grad_fq)
+            else:
+                grad_fq = np.ascontiguousarray(grad_fq, dtype=np.float32)
+        rows = grad_fq.shape[0] * grad_fq.shape[1]
+        if out is None:
+            out = np.empty(tuple(grad_fq.shape[:-1]) + (n_r,), np.float64)
+        check(self.lib.pyiid_grad_pdf(self.handle, _ptr(grad_fq), rows, _ptr(out), flags))
+        return out
+
+    # --- potentials --------------------------------------------------------
+    def potential(self, potential, gobs, gcalc):
+        gobs = np.ascontiguousarray(gobs, dtype=np.float64)
+        gcalc = np.ascontiguousarray(gcalc, dtype=np.float64)
+        v = ctypes.c_double()
+        s = ctypes.c_double()
+        check(self.lib.pyiid_potential(self.handle, potential, _ptr(gobs), _ptr(gcalc),
+                                       len(gobs), ctypes.byref(v), ctypes.byref(s), 0))
+        return v.value, s.value
+
+    def potential_grad(self, potential, grad_gcalc, gobs, gcalc):
+        gobs = np.ascontiguousarray(gobs, dtype=np.float64)
+        gcalc = np.ascontiguousarray(gcalc, dtype=np.float64)
+        gg = np.ascontiguousarray(grad_gcalc, dtype=np.float64)
+        rows = int(np.prod(gg.shape[:-1]))
+        out = np.empty(gg.shape[:-1], np.float64)
+        check(self.lib.pyiid_potential_grad(self.handle, potential, _ptr(gg), rows, _ptr(gobs),
+                                            _ptr(gcalc), len(gobs), _ptr(out), 0))
+        return out
+
+    def pdf_energy_forces(self, slot, qbin, gobs, potential, conv, n_atoms, forces=True):
+        gobs = np.ascontiguousarray(gobs, dtype=np.float64)
+        e = ctypes.c_double()
+        f = np.empty((n_atoms, 3), np.float64) if forces else None
+        check(self.lib.pyiid_pdf_energy_forces(self.handle, slot, float(qbin), _ptr(gobs),
+                                               potential, float(conv), ctypes.byref(e), _ptr(f),
+                                               0))
+        return e.value, f
+
+    def fq_energy_forces(self, slot, qbin, q_first, gobs, potential, conv, n_atoms, forces=True):
+        gobs = np.ascontiguousarray(gobs, dtype=np.float64)
+        e = ctypes.c_double()
+        f = np.empty((n_atoms, 3), np.float64) if forces else None
+        check(self.lib.pyiid_fq_energy_forces(self.handle, slot, float(qbin), int(q_first),
+                                              _ptr(gobs), potential, float(conv),
+                                              ctypes.byref(e), _ptr(f), 0))
+        return e.value, f

@@ -1,0 +1,519 @@
+// Debye scattering kernels for gfx950 (CDNA4, wave64).
+//
+//  * fused_grad_kernel: ordered pairs (i, j), one lane per atom i, a wave
+//    covers 64 atoms x kGradM consecutive Q bins; j atoms are staged in LDS
+//    and read as wave-wide broadcasts.  Per (i, j) the geometry, the phase
+//    theta0 = Q0 r and the step dtheta = dq r are formed in FP64 and reduced
+//    with a Cody-Waite split of pi/2; sin/cos come from Taylor polynomials in
+//    FP32.  Along Q the phasor (cos, sin) is advanced by the exact rotation
+//    by dtheta (angle-addition recurrence), so the inner loop is FMA only.
+//    Each lane accumulates F and the three gradient components for its atom;
+//    no atomics, no K x Q intermediates.
+//  * fq_kernel: unordered pairs (j < i) for F(Q) alone; omega_m = sin(Q_m r)/r
+//    follows the three-term Chebyshev recurrence w_{m+1} = 2cos(dtheta) w_m -
+//    w_{m-1}, two VALU ops per pair and Q bin.
+//  * partial sums leave each wave in FP64 and are reduced across tiles in
+//    FP64 by small deterministic reduction kernels.
+//
+// Reference semantics (pyIID): cpu_flat.py:77-153, cpu_experimental.py:8-15,
+// flat_multi_cpu_wrap.py:22-102, gpu_flat.py:11-299.
+#include "internal.h"
+
+namespace pyiid {
+namespace {
+
+using f4v = float __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// sin and cos of x (radians, x >= 0, |x| < 1e6).  Cody-Waite reduction by
+// pi/2 in FP64 (two-term split, FMA), then Taylor series on |r| <= pi/4:
+// sin through r^9 and cos through r^10 (truncation < 2e-9).
+__device__ __forceinline__ void sincos_red(double x, float& s, float& c) {
+  constexpr double kTwoOverPi = 0.63661977236758134308;
+  constexpr double kPio2Hi = 1.5707963267948966192;
+  constexpr double kPio2Lo = 6.123233995736766036e-17;
+  const double n = rint(x * kTwoOverPi);
+  double r = fma(-n, kPio2Hi, x);
+  r = fma(-n, kPio2Lo, r);
+  const float rf = static_cast<float>(r);
+  const float r2 = rf * rf;
+  float ps = fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+  ps = fmaf(r2, ps, 1.0f / 120.0f);
+  ps = fmaf(r2, ps, -1.0f / 6.0f);
+  const float sp = fmaf(rf * r2, ps, rf);
+  float pc = fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+  pc = fmaf(r2, pc, -1.0f / 720.0f);
+  pc = fmaf(r2, pc, 1.0f / 24.0f);
+  pc = fmaf(r2, pc, -0.5f);
+  const float cp = fmaf(r2, pc, 1.0f);
+  const int q = static_cast<int>(n) & 3;
+  const bool swap = q & 1;
+  float so = swap ? cp : sp;
+  float co = swap ? sp : cp;
+  s = (q & 2) ? -so : so;
+  c = ((q + 1) & 2) ? -co : co;
+}
+
+struct PairGeom {
+  float inv_r;   // 1/r (0 for excluded pairs)
+  float dpx, dpy, dpz;  // (q_i - q_j) / r^2
+  double r;      // |q_i - q_j| (0 for excluded pairs)
+};
+
+// Geometry of the pair (pi, pj).  `keep` false zeroes the pair's weights so
+// it contributes nothing (self pair, padding lanes, j >= i in F-only).
+__device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& pj, bool keep) {
+  const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+  const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+  keep = keep && (r2 > 0.0);
+  // FP32 square root refined by one Newton step in FP64 (relative error ~1e-14).
+  const float rf = __builtin_amdgcn_sqrtf(static_cast<float>(r2));
+  const double rf64 = static_cast<double>(rf);
+  const double rn = rf64 + fma(-rf64, rf64, r2) * static_cast<double>(0.5f * __builtin_amdgcn_rcpf(rf));
+  PairGeom g;
+  g.r = keep ? rn : 0.0;
+  const float inv_r = keep ? __builtin_amdgcn_rcpf(static_cast<float>(rn)) : 0.0f;
+  const float inv_r2 = inv_r * inv_r;
+  g.inv_r = inv_r;
+  g.dpx = static_cast<float>(dx) * inv_r2;
+  g.dpy = static_cast<float>(dy) * inv_r2;
+  g.dpz = static_cast<float>(dz) * inv_r2;
+  return g;
+}
+
+
+// Shared memory of the Debye kernels: the j tile during the pair loop, then
+// reused as the transpose buffer of the end-of-kernel lane reduction.
+constexpr int kRedRow = 65;  // 64 lanes + 1 pad (bank spread)
+constexpr int kRedM = 16;    // Q bins reduced per pass
+constexpr int kSmemBytes = (kWaves * kRedM * kRedRow * 4 > kJTile * 32) ? kWaves * kRedM * kRedRow * 4
+                                                                      : kJTile * 32;
+
+// Sum acc[m] over the wave's 64 lanes in FP64 (through LDS); returns, in lane
+// L < M, the total of bin L.  Must be reached by every wave of the block.
+template <int M>
+__device__ __forceinline__ double lane_reduce_f64(const float (&acc)[M], unsigned char* smem,
+                                                  int wave, int lane) {
+  float* red = reinterpret_cast<float*>(smem) + wave * kRedM * kRedRow;
+  double mine = 0.0;
+#pragma unroll
+  for (int m0 = 0; m0 < M; m0 += kRedM) {
+    __syncthreads();
+#pragma unroll
+    for (int mm = 0; mm < kRedM; ++mm) red[mm * kRedRow + lane] = acc[m0 + mm];
+    __syncthreads();
+    if (lane >= m0 && lane < m0 + kRedM) {
+      const float* row = red + (lane - m0) * kRedRow;
+      double t = 0.0;
+      for (int l = 0; l < 64; ++l) t += static_cast<double>(row[l]);
+      mine = t;
+    }
+  }
+  return mine;
+}
+
+// ---------------------------------------------------------------------------
+// Fused F + gradient kernel (ordered pairs).
+//   grid: x = i tile (64 atoms), y = Q group (kWaves waves x M bins), z = j split
+//   gpart[s][i][c][q]   float  sum_j t_ij(q) (q_i - q_j)_c / r^2   (ff-weighted)
+//   fpart[s][itile][q]  double sum_{i in tile} sum_j ff sin(Q r)/r
+//   CONTRACT: instead of gpart, force_part[s * n_qchunk + qchunk][i][c] =
+//             sum_q u[q] * gpart(i, c, q)
+// ---------------------------------------------------------------------------
+template <int M, bool WEIGHTED, bool CONTRACT>
+__global__ __launch_bounds__(256) void fused_grad_kernel(
+    const AtomRec* __restrict__ pos, int n, const float* __restrict__ ftab, int n_elem, int nq,
+    double dq, int j_chunk, float* __restrict__ gpart, double* __restrict__ fpart,
+    const float* __restrict__ u, float* __restrict__ force_part) {
+  __shared__ __attribute__((aligned(32))) unsigned char smem[kSmemBytes];
+  AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
+  __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * 64 + lane;
+  const int qchunk = blockIdx.y * kWaves + wave;
+  const int q0 = qchunk * M;
+  const int s = blockIdx.z;
+  const int j_begin = s * j_chunk;
+  const int j_end = min(n, j_begin + j_chunk);
+  const bool lane_ok = i < n;
+  const AtomRec pi = pos[lane_ok ? i : n - 1];
+
+  float qv[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) qv[m] = uniform_f(static_cast<float>(static_cast<double>(q0 + m) * dq));
+
+  float fi[WEIGHTED ? M : 1];
+  if constexpr (WEIGHTED) {
+    const int ei = static_cast<int>(pi.w);
+#pragma unroll
+    for (int m = 0; m < M; ++m) fi[m] = (q0 + m < nq) ? ftab[(size_t)ei * nq + q0 + m] : 0.0f;
+    const int qb = blockIdx.y * kWaves * M;
+    for (int t = threadIdx.x; t < kMaxElem * kWaves * M; t += blockDim.x) {
+      const int e = t / (kWaves * M), qq = t % (kWaves * M);
+      fl[e][qq] = (e < n_elem && qb + qq < nq) ? ftab[(size_t)e * nq + qb + qq] : 0.0f;
+    }
+  }
+
+  float aF[M], ax[M], ay[M], az[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = 0.0f;
+
+  const double q0d = static_cast<double>(q0);
+  for (int jt = j_begin; jt < j_end; jt += kJTile) {
+    __syncthreads();
+    if (jt + (int)threadIdx.x < j_end) tile[threadIdx.x] = pos[jt + threadIdx.x];
+    __syncthreads();
+    const int cnt = min(kJTile, j_end - jt);
+    if (q0 >= nq) continue;  // wave-uniform
+    for (int jj = 0; jj < cnt; ++jj) {
+      const AtomRec pj = tile[jj];
+      const PairGeom g = pair_geom(pi, pj, lane_ok);
+      const double dth = dq * g.r;
+      float sn, cs, s1, c1;
+      sincos_red(q0d * dth, sn, cs);
+      sincos_red(dth, s1, c1);
+      if constexpr (WEIGHTED) {
+        const int ej = static_cast<int>(pj.w);
+        const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
+#pragma unroll
+        for (int m4 = 0; m4 < M / 4; ++m4) {
+          const f4v fj = fj4[m4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int m = m4 * 4 + e;
+            const float ff = fi[m] * fj[e];
+            const float w = sn * g.inv_r;
+            const float t = fmaf(qv[m], cs, -w) * ff;
+            aF[m] = fmaf(w, ff, aF[m]);
+            ax[m] = fmaf(t, g.dpx, ax[m]);
+            ay[m] = fmaf(t, g.dpy, ay[m]);
+            az[m] = fmaf(t, g.dpz, az[m]);
+            const float sn2 = fmaf(sn, c1, cs * s1);
+            cs = fmaf(cs, c1, -sn * s1);
+            sn = sn2;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const float w = sn * g.inv_r;
+          const float t = fmaf(qv[m], cs, -w);
+          aF[m] += w;
+          ax[m] = fmaf(t, g.dpx, ax[m]);
+          ay[m] = fmaf(t, g.dpy, ay[m]);
+          az[m] = fmaf(t, g.dpz, az[m]);
+          const float sn2 = fmaf(sn, c1, cs * s1);
+          cs = fmaf(cs, c1, -sn * s1);
+          sn = sn2;
+        }
+      }
+    }
+  }
+
+  // F partial of this wave (FP64 lane reduction), lane m keeps bin q0 + m.
+  const double fmine = lane_reduce_f64<M>(aF, smem, wave, lane);
+  const int n_itile = gridDim.x;
+  if (lane < M && q0 + lane < nq)
+    fpart[((size_t)s * n_itile + blockIdx.x) * nq + q0 + lane] = fmine;
+
+  if constexpr (CONTRACT) {
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const float um = (q0 + m < nq) ? u[q0 + m] : 0.0f;
+      fx = fmaf(um, ax[m], fx);
+      fy = fmaf(um, ay[m], fy);
+      fz = fmaf(um, az[m], fz);
+    }
+    if (lane_ok) {
+      const size_t p = (size_t)s * (gridDim.y * kWaves) + qchunk;
+      float* dst = force_part + (p * n + i) * 3;
+      dst[0] = fx;
+      dst[1] = fy;
+      dst[2] = fz;
+    }
+  } else {
+    if (lane_ok && q0 < nq) {
+      float* gx = gpart + (((size_t)s * n + i) * 3) * nq + q0;
+      float* gy = gx + nq;
+      float* gz = gy + nq;
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        if (q0 + m < nq) {
+          gx[m] = ax[m];
+          gy[m] = ay[m];
+          gz[m] = az[m];
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// F-only kernel (unordered pairs j < i), Chebyshev recurrence on omega.
+//   grid: x = work item (i tile, j range), y = Q group
+//   fpart[item][q] double
+// ---------------------------------------------------------------------------
+template <int M, bool WEIGHTED>
+__global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos, int n,
+                                                 const float* __restrict__ ftab, int n_elem,
+                                                 int nq, double dq,
+                                                 const FqItem* __restrict__ items,
+                                                 double* __restrict__ fpart) {
+  __shared__ __attribute__((aligned(32))) unsigned char smem[kSmemBytes];
+  AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
+  __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
+
+  const FqItem it = items[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = it.itile * 64 + lane;
+  const int q0 = (blockIdx.y * kWaves + wave) * M;
+  const bool lane_ok = i < n;
+  const AtomRec pi = pos[lane_ok ? i : n - 1];
+
+  float fi[WEIGHTED ? M : 1];
+  if constexpr (WEIGHTED) {
+    const int ei = static_cast<int>(pi.w);
+#pragma unroll
+    for (int m = 0; m < M; ++m) fi[m] = (q0 + m < nq) ? ftab[(size_t)ei * nq + q0 + m] : 0.0f;
+    const int qb = blockIdx.y * kWaves * M;
+    for (int t = threadIdx.x; t < kMaxElem * kWaves * M; t += blockDim.x) {
+      const int e = t / (kWaves * M), qq = t % (kWaves * M);
+      fl[e][qq] = (e < n_elem && qb + qq < nq) ? ftab[(size_t)e * nq + qb + qq] : 0.0f;
+    }
+  }
+
+  float acc[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) acc[m] = 0.0f;
+
+  const double q0d = static_cast<double>(q0);
+  for (int jt = it.jbeg; jt < it.jend; jt += kJTile) {
+    __syncthreads();
+    if (jt + (int)threadIdx.x < it.jend) tile[threadIdx.x] = pos[jt + threadIdx.x];
+    __syncthreads();
+    const int cnt = min(kJTile, it.jend - jt);
+    if (q0 >= nq) continue;
+    for (int jj = 0; jj < cnt; ++jj) {
+      const AtomRec pj = tile[jj];
+      const PairGeom g = pair_geom(pi, pj, lane_ok && (jt + jj < i));
+      const double dth = dq * g.r;
+      float sn, cs, s1, c1;
+      sincos_red(q0d * dth, sn, cs);
+      sincos_red(dth, s1, c1);
+      const float k = 2.0f * c1;
+      float w0 = sn * g.inv_r;
+      float w1 = fmaf(sn, c1, cs * s1) * g.inv_r;
+      if constexpr (WEIGHTED) {
+        const int ej = static_cast<int>(pj.w);
+        const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
+#pragma unroll
+        for (int m4 = 0; m4 < M / 4; ++m4) {
+          const f4v fj = fj4[m4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int m = m4 * 4 + e;
+            acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
+            const float w2 = fmaf(k, w1, -w0);
+            w0 = w1;
+            w1 = w2;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          acc[m] += w0;
+          const float w2 = fmaf(k, w1, -w0);
+          w0 = w1;
+          w1 = w2;
+        }
+      }
+    }
+  }
+
+  const double mine = lane_reduce_f64<M>(acc, smem, wave, lane);
+  if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine;
+}
+
+__global__ void reduce_rows_kernel(const double* __restrict__ part, int n_part, int nq,
+                                   const double* __restrict__ scale, double* __restrict__ out64,
+                                   float* __restrict__ out32) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  double acc = 0.0;
+  for (int p = 0; p < n_part; ++p) acc += part[(size_t)p * nq + q];
+  const double v = acc * scale[q];
+  if (out64) out64[q] = v;
+  if (out32) out32[q] = static_cast<float>(v);
+}
+
+template <typename T>
+__global__ void reduce_grad_kernel(const float* __restrict__ gpart, int n_split, int64_t total,
+                                   int nq, const double* __restrict__ gscale, T* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int s = 0; s < n_split; ++s) acc += static_cast<double>(gpart[(size_t)s * total + e]);
+    out[e] = static_cast<T>(acc * gscale[e % nq]);
+  }
+}
+
+__global__ void reduce_force_kernel(const float* __restrict__ fpart, int n_part, int64_t total,
+                                    double conv, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int p = 0; p < n_part; ++p) acc += static_cast<double>(fpart[(size_t)p * total + e]);
+    out[e] = acc * conv;
+  }
+}
+
+// na(q) = N * mean_{i>j} f_i f_j = ((sum_e n_e f_e)^2 - sum_e n_e f_e^2) / (N - 1).
+// Unweighted kernels (one element) accumulate without f_i f_j = f^2, so their
+// scales carry w = f^2; weighted kernels carry w = 1.  0/0 -> 0 (nan_to_num).
+__global__ void norm_kernel(const float* __restrict__ ftab, const int64_t* __restrict__ counts,
+                            int n_elem, int nq, int64_t n, int weighted, double* na,
+                            double* fsu, double* fso, double* gs) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  double s1 = 0.0, s2 = 0.0;
+  for (int e = 0; e < n_elem; ++e) {
+    const double f = ftab[(size_t)e * nq + q];
+    const double c = static_cast<double>(counts[e]);
+    s1 += c * f;
+    s2 += c * f * f;
+  }
+  double nav = 0.0, w = 0.0;
+  if (n >= 2) {
+    nav = (s1 * s1 - s2) / static_cast<double>(n - 1);
+    const double f0 = ftab[q];
+    w = weighted ? 1.0 : f0 * f0;
+  }
+  const bool ok = (n >= 2) && (nav != 0.0);
+  if (na) na[q] = (n >= 2) ? nav : __builtin_nan("");
+  fsu[q] = ok ? 2.0 * w / nav : 0.0;
+  fso[q] = ok ? w / nav : 0.0;
+  gs[q] = ok ? -w / nav : 0.0;
+}
+
+// Flat pair k = j + i(i-1)/2 (i > j): d = q_i - q_j, r = |d| (float32 out).
+__global__ void pair_geometry_kernel(const AtomRec* __restrict__ pos, int64_t k0, int64_t kc,
+                                     float* __restrict__ d, float* __restrict__ r) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < kc;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = k0 + t;
+    int64_t i = static_cast<int64_t>((1.0 + sqrt(1.0 + 8.0 * static_cast<double>(k))) * 0.5);
+    if (i * (i - 1) / 2 > k) --i;
+    if ((i + 1) * i / 2 <= k) ++i;
+    const int64_t j = k - i * (i - 1) / 2;
+    const AtomRec a = pos[i], b = pos[j];
+    const double dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+    const float fx = static_cast<float>(dx), fy = static_cast<float>(dy),
+                fz = static_cast<float>(dz);
+    if (d) {
+      d[3 * t + 0] = fx;
+      d[3 * t + 1] = fy;
+      d[3 * t + 2] = fz;
+    }
+    if (r) r[t] = static_cast<float>(sqrt(fma(dx, dx, fma(dy, dy, dz * dz))));
+  }
+}
+
+int grid_for(int64_t total, int block) {
+  int64_t g = (total + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<int>(g);
+}
+
+}  // namespace
+
+hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n, const float* ftab,
+                             int n_elem, int nq, double dq, int n_split, int j_chunk,
+                             bool weighted, float* gpart, double* fpart, const float* u,
+                             float* force_part) {
+  constexpr int M = kGradM;
+  dim3 grid((n + 63) / 64, (nq + kWaves * M - 1) / (kWaves * M), n_split);
+  dim3 block(256);
+  const bool contract = (u != nullptr);
+  if (weighted) {
+    if (contract)
+      hipLaunchKernelGGL((fused_grad_kernel<M, true, true>), grid, block, 0, st, pos, n, ftab,
+                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+    else
+      hipLaunchKernelGGL((fused_grad_kernel<M, true, false>), grid, block, 0, st, pos, n, ftab,
+                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+  } else {
+    if (contract)
+      hipLaunchKernelGGL((fused_grad_kernel<M, false, true>), grid, block, 0, st, pos, n, ftab,
+                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+    else
+      hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, n, ftab,
+                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n, const float* ftab, int n_elem,
+                     int nq, double dq, const FqItem* items, int n_items, bool weighted,
+                     double* fpart) {
+  constexpr int M = kFqM;
+  dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
+  if (weighted)
+    hipLaunchKernelGGL((fq_kernel<M, true>), grid, dim3(256), 0, st, pos, n, ftab, n_elem, nq,
+                       dq, items, fpart);
+  else
+    hipLaunchKernelGGL((fq_kernel<M, false>), grid, dim3(256), 0, st, pos, n, ftab, n_elem, nq,
+                       dq, items, fpart);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows(hipStream_t st, const double* part, int n_part, int nq,
+                              const double* scale, double* out_f64, float* out_f32) {
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, part, n_part,
+                     nq, scale, out_f64, out_f32);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, int64_t n, int nq,
+                              const double* gscale, void* out, bool out_f64) {
+  const int64_t total = n * 3 * nq;
+  if (out_f64)
+    hipLaunchKernelGGL(reduce_grad_kernel<double>, dim3(grid_for(total, 256)), dim3(256), 0, st,
+                       gpart, n_split, total, nq, gscale, static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(reduce_grad_kernel<float>, dim3(grid_for(total, 256)), dim3(256), 0, st,
+                       gpart, n_split, total, nq, gscale, static_cast<float*>(out));
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_force(hipStream_t st, const float* fpart, int n_part, int64_t n,
+                               double conv, double* out) {
+  const int64_t total = n * 3;
+  hipLaunchKernelGGL(reduce_force_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, fpart,
+                     n_part, total, conv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm(hipStream_t st, const float* ftab, const int64_t* counts, int n_elem,
+                       int nq, int64_t n, bool weighted, double* na, double* fsu, double* fso,
+                       double* gs) {
+  hipLaunchKernelGGL(norm_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, ftab, counts, n_elem,
+                     nq, n, weighted ? 1 : 0, na, fsu, fso, gs);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_geometry(hipStream_t st, const AtomRec* pos, int64_t k_offset,
+                                int64_t k_count, float* d, float* r) {
+  hipLaunchKernelGGL(pair_geometry_kernel, dim3(grid_for(k_count, 256)), dim3(256), 0, st, pos,
+                     k_offset, k_count, d, r);
+  return hipGetLastError();
+}
+
+}  // namespace pyiid

@@ -1,0 +1,255 @@
+// F(Q) -> G(r) transform on the FP32 matrix cores, and the FP64 potential
+// (Rw / chi^2) reductions with their adjoint vectors.
+//
+// The reference transform (kernels/master_kernel.py:39-203) is an odd-extended
+// zero-padded inverse FFT followed by linear interpolation onto the r grid;
+// it is exactly G = T . F with a dense R x Q matrix T (built on the host in
+// FP64, stored here as FP32).  grad G = grad F . T^T is the only dense
+// contraction of the path and runs on v_mfma_f32_16x16x4_f32.
+#include "internal.h"
+
+namespace pyiid {
+namespace {
+
+using v4f = float __attribute__((ext_vector_type(4)));
+
+// C[m][n] = sum_k A[m][k] * B[n][k]; block tile 64 x 64, 4 waves of 32 x 32,
+// each wave 2 x 2 MFMA 16x16x4 tiles; K staged through LDS 16 at a time.
+// MFMA f32 16x16x4 operand layout (lane l): A[l%16][l/16], B[l/16][l%16];
+// result register v holds C[4*(l/16)+v][l%16].
+template <bool A64>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(const void* __restrict__ a_raw, int64_t m,
+                                                      int k, const float* __restrict__ b, int n,
+                                                      double* __restrict__ c64,
+                                                      float* __restrict__ c32) {
+  constexpr int KS = 16;
+  __shared__ float as[64][KS + 1];
+  __shared__ float bs[64][KS + 1];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t bm = static_cast<int64_t>(blockIdx.y) * 64;
+  const int bn = blockIdx.x * 64;
+  v4f acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < k; k0 += KS) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = threadIdx.x + e * 256;
+      const int row = idx / KS, kk = idx % KS;
+      const int64_t ar = bm + row;
+      float av = 0.0f;
+      if (ar < m && k0 + kk < k) {
+        if constexpr (A64)
+          av = static_cast<float>(static_cast<const double*>(a_raw)[ar * k + k0 + kk]);
+        else
+          av = static_cast<const float*>(a_raw)[ar * k + k0 + kk];
+      }
+      as[row][kk] = av;
+      const int br = bn + row;
+      bs[row][kk] = (br < n && k0 + kk < k) ? b[(int64_t)br * k + k0 + kk] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS; kk += 4) {
+      float af[2], bf[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) af[x] = as[wm * 32 + x * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int y = 0; y < 2; ++y) bf[y] = bs[wn * 32 + y * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[x], bf[y], acc[x][y], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = bm + wm * 32 + x * 16 + 4 * (lane >> 4) + v;
+        const int col = bn + wn * 32 + y * 16 + (lane & 15);
+        if (row < m && col < n) {
+          if (c64) c64[row * n + col] = static_cast<double>(acc[x][y][v]);
+          if (c32) c32[row * n + col] = acc[x][y][v];
+        }
+      }
+}
+
+__device__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  const int nw = blockDim.x >> 6;
+  for (int w = 0; w < nw; ++w) t += sh[w];
+  return t;
+}
+
+// Single block.  Rw: master_kernel.py:206-236; chi^2: :239-266; the adjoint
+// w is d(potential)/d(gcalc) as get_grad_rw / get_grad_chi_sq apply it
+// (:293-375): grad = gradG . w with
+//   w = c * (s * res + (g.res)/(g.g) * (o - 2 a g)),
+//   c = -value/|res|^2 (Rw) or -2 (chi^2), res = o - s g, a = optimal scale.
+__global__ __launch_bounds__(1024) void potential_kernel(int potential,
+                                                         const double* __restrict__ o,
+                                                         const double* __restrict__ g, int n,
+                                                         double* __restrict__ out5,
+                                                         double* __restrict__ w) {
+  __shared__ double sh[16];
+  double go = 0.0, gg = 0.0, oo = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    go += g[t] * o[t];
+    gg += g[t] * g[t];
+    oo += o[t] * o[t];
+  }
+  go = block_sum(go, sh);
+  gg = block_sum(gg, sh);
+  oo = block_sum(oo, sh);
+  const double a = go / gg;
+  const double s = (a <= 0.0) ? 1.0 : a;
+  double rr = 0.0, gr = 0.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const double res = o[t] - s * g[t];
+    rr += res * res;
+    gr += g[t] * res;
+  }
+  rr = block_sum(rr, sh);
+  gr = block_sum(gr, sh);
+  double value, coef;
+  if (potential == 0) {
+    value = (a <= 0.0) ? 1.0 : sqrt(rr / oo);
+    coef = -value / rr;
+  } else {
+    value = rr;
+    coef = -2.0;
+  }
+  if (w) {
+    const double gfac = gr / gg;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const double res = o[t] - s * g[t];
+      w[t] = coef * (s * res + gfac * (o[t] - 2.0 * a * g[t]));
+    }
+  }
+  if (threadIdx.x == 0) {
+    out5[0] = value;
+    out5[1] = s;
+    out5[2] = a;
+    out5[3] = rr;
+    out5[4] = gr;
+  }
+}
+
+__global__ void gemv_rows_kernel(const double* __restrict__ x, int64_t rows, int n,
+                                 const double* __restrict__ w, double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  double acc = 0.0;
+  for (int t = lane; t < n; t += 64) acc += x[row * n + t] * w[t];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) y[row] = acc;
+}
+
+__global__ void adjoint_kernel(const float* __restrict__ t, int n_r, int nq,
+                               const double* __restrict__ w, const double* __restrict__ gscale,
+                               float* __restrict__ u) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  double acc = 0.0;
+  for (int r = 0; r < n_r; ++r) acc += static_cast<double>(t[(size_t)r * nq + q]) * w[r];
+  u[q] = static_cast<float>(acc * gscale[q]);
+}
+
+__global__ void adjoint_fq_kernel(int nq, int q_first, const double* __restrict__ w,
+                                  const double* __restrict__ gscale, float* __restrict__ u) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  u[q] = (q >= q_first) ? static_cast<float>(w[q - q_first] * gscale[q]) : 0.0f;
+}
+
+__global__ void f64_to_f32_kernel(const double* __restrict__ x, int64_t n, float* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = static_cast<float>(x[e]);
+}
+
+__global__ void f32_to_f64_kernel(const float* __restrict__ x, int64_t n, double* __restrict__ y) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    y[e] = static_cast<double>(x[e]);
+}
+
+int blocks_for(int64_t total, int block) {
+  int64_t g = (total + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return static_cast<int>(g);
+}
+
+}  // namespace
+
+hipError_t launch_gemm_nt(hipStream_t st, const void* a, bool a_f64, int64_t m, int k,
+                          const float* b, int n, double* c_f64, float* c_f32) {
+  const int64_t gy = (m + 63) / 64;
+  if (gy > 2147483647LL) return hipErrorInvalidValue;
+  dim3 grid((n + 63) / 64, static_cast<unsigned>(gy));
+  if (a_f64)
+    hipLaunchKernelGGL(gemm_nt_kernel<true>, grid, dim3(256), 0, st, a, m, k, b, n, c_f64, c_f32);
+  else
+    hipLaunchKernelGGL(gemm_nt_kernel<false>, grid, dim3(256), 0, st, a, m, k, b, n, c_f64, c_f32);
+  return hipGetLastError();
+}
+
+hipError_t launch_potential(hipStream_t st, int potential, const double* gobs,
+                            const double* gcalc, int n, double* out5, double* w) {
+  hipLaunchKernelGGL(potential_kernel, dim3(1), dim3(1024), 0, st, potential, gobs, gcalc, n, out5,
+                     w);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemv_rows(hipStream_t st, const double* x, int64_t rows, int n, const double* w,
+                            double* y) {
+  const int64_t blocks = (rows + 3) / 4;
+  hipLaunchKernelGGL(gemv_rows_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, x,
+                     rows, n, w, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint(hipStream_t st, const float* t, int n_r, int nq, const double* w,
+                          const double* gscale, float* u) {
+  hipLaunchKernelGGL(adjoint_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, t, n_r, nq, w,
+                     gscale, u);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint_fq(hipStream_t st, int nq, int q_first, const double* w,
+                             const double* gscale, float* u) {
+  hipLaunchKernelGGL(adjoint_fq_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, nq, q_first, w,
+                     gscale, u);
+  return hipGetLastError();
+}
+
+hipError_t launch_f64_to_f32(hipStream_t st, const double* x, int64_t n, float* y) {
+  hipLaunchKernelGGL(f64_to_f32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, y);
+  return hipGetLastError();
+}
+
+hipError_t launch_f32_to_f64(hipStream_t st, const float* x, int64_t n, double* y) {
+  hipLaunchKernelGGL(f32_to_f64_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, n, y);
+  return hipGetLastError();
+}
+
+}  // namespace pyiid

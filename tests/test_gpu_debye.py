@@ -1,0 +1,109 @@
+"""GPU parity of the Debye kernels against the FP64 oracle (through the C ABI).
+
+Tolerance (from the target of this path): max|x - x*| / max|x*| <= 1e-5 for
+F(Q) and the gradient, with float32 positions on both sides.
+"""
+import numpy as np
+import pytest
+
+from oracle import scatter as oracle
+from pyiid_amd import _lib, structures
+from pyiid_amd.formfactors import element_table
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / np.max(np.abs(b)))
+
+
+def load(ctx, pos, numbers, qbin, nq, slot=_lib.SLOT_FQ):
+    table, index, _ = element_table(numbers, qbin, nq)
+    ctx.set_positions(pos)
+    ctx.set_elements(index)
+    ctx.set_scatter(slot, table)
+    return table[index]
+
+
+def cubo():
+    return structures.jitter(structures.cuboctahedron(4.08), 0.05, seed=0).astype(np.float32)
+
+
+@pytest.mark.parametrize("qbin,nq", [(0.1, 250), (0.0756865, 330), (0.02, 1250)])
+def test_fq_cuboctahedron(ctx, qbin, nq):
+    pos = cubo()
+    per_atom = load(ctx, pos, np.full(len(pos), 79), qbin, nq)
+    got = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    want = oracle.fq(pos, per_atom, qbin)
+    print("fq N=%d nq=%d rel %.2e" % (len(pos), nq, rel(got, want)))
+    assert rel(got, want) < TOL
+
+
+@pytest.mark.parametrize("qbin,nq", [(0.1, 250), (0.02, 1250)])
+def test_grad_fq_cuboctahedron(ctx, qbin, nq):
+    pos = cubo()
+    per_atom = load(ctx, pos, np.full(len(pos), 79), qbin, nq)
+    fq_out = np.empty(nq, np.float32)
+    got = ctx.grad_fq(_lib.SLOT_FQ, qbin, len(pos), nq, fq_out=fq_out, f64=True)
+    want = oracle.grad_fq(pos, per_atom, qbin)
+    print("grad N=%d nq=%d rel %.2e" % (len(pos), nq, rel(got, want)))
+    assert rel(got, want) < TOL
+    assert rel(fq_out, oracle.fq(pos, per_atom, qbin)) < TOL
+
+
+def test_sphere_and_float32_output(ctx):
+    pos = structures.jitter(structures.fcc_sphere(9.0), 0.05, seed=3).astype(np.float32)
+    n, qbin, nq = len(pos), 0.02, 1250
+    per_atom = load(ctx, pos, np.full(n, 79), qbin, nq)
+    fq = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    qi = np.arange(0, nq, 7)
+    assert rel(fq[qi], oracle.fq(pos, per_atom, qbin, q_index=qi)) < TOL
+    g32 = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq)
+    assert g32.dtype == np.float32
+    rows = np.arange(0, n, 5)
+    want = oracle.grad_fq(pos, per_atom, qbin, q_index=qi, atom_index=rows)
+    assert rel(g32[rows][:, :, qi], want) < TOL
+
+
+def test_alloy_weighted(ctx):
+    base = structures.fcc_nearest(300, a=3.724)
+    pos = structures.jitter(base, 0.05, seed=5).astype(np.float32)
+    numbers = structures.random_alloy(len(pos), 78, 28, seed=7)
+    qbin, nq = 0.05, 500
+    per_atom = load(ctx, pos, numbers, qbin, nq)
+    fq = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    assert rel(fq, oracle.fq(pos, per_atom, qbin)) < TOL
+    g = ctx.grad_fq(_lib.SLOT_FQ, qbin, len(pos), nq, f64=True)
+    assert rel(g, oracle.grad_fq(pos, per_atom, qbin)) < TOL
+    na = ctx.normalization(_lib.SLOT_FQ, nq)
+    np.testing.assert_allclose(na, oracle.normalization(per_atom), rtol=1e-12)
+
+
+def test_pair_geometry(ctx):
+    pos = cubo()
+    load(ctx, pos, np.full(len(pos), 79), 0.1, 250)
+    k = len(pos) * (len(pos) - 1) // 2
+    d, r = ctx.pair_geometry(0, k)
+    dw, rw = oracle.pair_geometry(pos)
+    np.testing.assert_allclose(d, dw, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(r, rw, rtol=1e-6)
+    d2, r2 = ctx.pair_geometry(100, 200)
+    np.testing.assert_array_equal(d2, d[100:300])
+    np.testing.assert_array_equal(r2, r[100:300])
+
+
+@pytest.mark.parametrize("n", [0, 1, 2])
+def test_tiny_structures(ctx, n):
+    pos = np.array([[0, 0, 0], [2.5, 0.3, 0.1]], np.float32)[:n]
+    qbin, nq = 0.1, 250
+    per_atom = load(ctx, pos, np.full(n, 79), qbin, nq)
+    fq = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    g = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, f64=True)
+    if n < 2:
+        assert not np.any(fq) and not np.any(g)
+    else:
+        r = np.linalg.norm(pos[1].astype(np.float64) - pos[0])
+        Q = oracle.q_values(qbin, nq)
+        assert rel(fq, np.sin(Q * r) / r) < TOL  # dimer closed form
+        assert rel(g, oracle.grad_fq(pos, per_atom, qbin)) < TOL
