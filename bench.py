@@ -1,0 +1,187 @@
+"""Benchmark of the Debye F(Q) + gradient hot path on one MI355X.
+
+Workload: Au FCC sphere of radius 30 A (6,699 atoms, a = 4.08 A) with seeded
+sigma = 0.05 A Gaussian jitter, float32 positions, Q = 0 ... 24.98 A^-1 in
+steps of 0.02 (1,250 bins).  One step = one F(Q) + grad F(Q) evaluation
+(pyiid_grad_fq with F output) with inputs resident in HBM and outputs written
+to device memory.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Debye F(Q)+grad pair·Q evals/s; HMC wall-s to target Rw "
+          "(surrogate vs exact)")
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector peak (published)
+FLOP_PER_PAIR_Q = 21       # F + grad, sin/cos counted as 1 FLOP, FMA as 2
+QBIN, NQ, RADIUS, LATTICE, SIGMA, SEED = 0.02, 1250, 30.0, 4.08, 0.05, 0
+
+
+def workload():
+    from pyiid_amd import structures
+    from pyiid_amd.formfactors import element_table
+    pos = structures.jitter(structures.fcc_sphere(RADIUS, LATTICE), SIGMA, SEED)
+    pos = pos.astype(np.float32)
+    table, index, _ = element_table(np.full(len(pos), 79), QBIN, NQ)
+    return pos, table, index
+
+
+def read_traffic(n_pairs_q):
+    """HBM bytes per launch of the fused kernel from the committed rocprof
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "fused_grad_au6699_q1250_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if "=" in line and not line.lstrip().startswith("#"):
+                k, v = line.split("=", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(pos, table, index, target_s=15.0):
+    """The oracle's C restatement of the reference 'flat' CPU algorithm on the
+    host cores, on a strided subset of the Q bins (cost is linear in bins)."""
+    from oracle import native
+    threads = max(1, min(16, os.cpu_count() or 1))
+    per_atom = table[index]
+    n = len(pos)
+    pairs = n * (n - 1) // 2
+    probe = np.array([NQ // 2], np.int32)
+    t0 = time.perf_counter()
+    native.flat_f32(pos, per_atom, QBIN, q_index=probe, nthreads=threads)
+    dt = time.perf_counter() - t0
+    nsel = int(max(2, min(NQ, target_s / max(dt, 1e-3))))
+    qi = np.unique(np.linspace(0, NQ - 1, nsel).round().astype(np.int32))
+    t0 = time.perf_counter()
+    native.flat_f32(pos, per_atom, QBIN, q_index=qi, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": pairs * len(qi) / dt,
+        "unit": "pair·Q evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "same structure, F+grad over all %d pairs at %d of the %d Q bins "
+                  "(evenly strided), %.1f s" % (pairs, len(qi), NQ, dt),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times (the known-size "
+                         "reference for the FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("copied 3 x %d bytes" % (4 * src.numel()))
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pyiid_amd import _lib
+
+    pos, table, index = workload()
+    n = len(pos)
+    pairs = n * (n - 1) // 2
+    ctx = _lib.Context(0)
+    ctx.set_positions(pos)
+    ctx.set_elements(index)
+    ctx.set_scatter(_lib.SLOT_FQ, table)
+    grad = torch.empty((n, 3, NQ), dtype=torch.float32, device="cuda")
+    fq = torch.empty(NQ, dtype=torch.float32, device="cuda")
+    flags = _lib.PYIID_DEVICE
+
+    def step():
+        ctx.grad_fq(_lib.SLOT_FQ, QBIN, n, NQ, out=grad.data_ptr(), fq_out=fq.data_ptr(),
+                    flags=flags)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ctx.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fused_ms, fused_n = ctx.kernel_time(_lib.KERNEL_FUSED)
+    ctx.enable_timing(False)
+    elapsed = t1 - t0
+    value = pairs * NQ * args.steps / elapsed
+    avg_s = fused_ms / max(1, fused_n) / 1e3
+    achieved = FLOP_PER_PAIR_Q * pairs * NQ / avg_s / 1e12
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair·Q evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "Au FCC sphere r=30 A (a=4.08 A, 6699 atoms), sigma=0.05 A seeded "
+                        "jitter (seed 0), float32 positions, Q 0-24.98 A^-1 step 0.02 "
+                        "(1250 bins); F(Q)+grad F(Q) per step",
+            "n_atoms": n,
+            "pairs": pairs,
+            "q_bins": NQ,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "fused_grad_kernel (F + grad, ordered pairs)",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": read_traffic(pairs * NQ),
+            "avg_launch_ms": avg_s * 1e3,
+            "launches": fused_n,
+            "flop_per_unit": FLOP_PER_PAIR_Q,
+        },
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(pos, table, index)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -92,7 +92,11 @@ def k_to_ij(k):
     return i, k - i * (i - 1) // 2
 
 
-def _as_f32_f64(x):
+def _as_f32_f64(x, cast=True):
+    """float32-rounded (as the reference casts positions and tables), widened
+    to float64; ``cast=False`` keeps float64 input exact (finite differences)."""
+    if not cast:
+        return np.asarray(x, dtype=np.float64)
     return np.asarray(x, dtype=np.float32).astype(np.float64)
 
 
@@ -110,9 +114,9 @@ def pair_geometry(q, k_offset=0, k_count=None):
     return d, np.sqrt(np.sum(d * d, axis=1))
 
 
-def pair_geometry_nxn(q):
+def pair_geometry_nxn(q, cast_f32=True):
     """Full ``d[i, j] = q_j - q_i`` and ``r[i, j]`` (``cpu_nxn.py:17-54``)."""
-    q = _as_f32_f64(q)
+    q = _as_f32_f64(q, cast_f32)
     d = q[None, :, :] - q[:, None, :]
     return d, np.sqrt(np.sum(d * d, axis=-1))
 
@@ -160,14 +164,14 @@ def _pair_chunks(n, chunk):
 # Debye F(Q): cpu_flat.py:77-114, cpu_atomics.py:60-78,
 # flat_multi_cpu_wrap.py:22-60
 # ---------------------------------------------------------------------------
-def fq(q, scatter, qbin, q_index=None, chunk_elems=1 << 22):
+def fq(q, scatter, qbin, q_index=None, chunk_elems=1 << 22, cast_f32=True):
     """Reduced structure factor F(Q) on the full grid from Q = 0.
 
     F(Q) = 2 / na(Q) * sum_{i>j} f_i f_j sin(Q r_ij) / r_ij with
     ``nan_to_num`` (``flat_multi_cpu_wrap.py:49-60``).  ``q_index`` restricts
     the evaluation to a subset of Q bins (the sum is independent per bin).
     """
-    q = _as_f32_f64(q)
+    q = _as_f32_f64(q, cast_f32)
     f = _as_f32_f64(scatter)
     n, nq = f.shape
     idx = np.arange(nq) if q_index is None else np.asarray(q_index)
@@ -183,14 +187,14 @@ def fq(q, scatter, qbin, q_index=None, chunk_elems=1 << 22):
     return 2.0 * _nan_to_num_div(acc, na)
 
 
-def fq_nxn(q, scatter, qbin):
+def fq_nxn(q, scatter, qbin, cast_f32=True):
     """F(Q) by the full N x N double sum (``cpu_nxn.py:57-111`` +
     ``nxn_cpu_wrap.py:206-266``): every ordered pair i != j once, no factor 2.
     An independent code path used to pin :func:`fq` (small N)."""
     f = _as_f32_f64(scatter)
     n, nq = f.shape
     Q = q_values(qbin, nq)
-    _, r = pair_geometry_nxn(q)
+    _, r = pair_geometry_nxn(q, cast_f32)
     off = ~np.eye(n, dtype=bool)
     rr = r[off]
     ii, jj = np.nonzero(off)
@@ -203,7 +207,7 @@ def fq_nxn(q, scatter, qbin):
 # flat_multi_cpu_wrap.py:63-102
 # ---------------------------------------------------------------------------
 def grad_fq(q, scatter, qbin, q_index=None, atom_index=None,
-            chunk_elems=1 << 22):
+            chunk_elems=1 << 22, cast_f32=True):
     """What ``get_grad_fq`` returns: (N, 3, Qb) array equal to -1/2 dF/dq.
 
     Per flat pair k=(i>j), d = q_i - q_j, g = f_i f_j (Q cos(Q r) - sin(Q r)/r)
@@ -212,7 +216,7 @@ def grad_fq(q, scatter, qbin, q_index=None, atom_index=None,
     factor 2 used for F (``flat_multi_cpu_wrap.py:93-102``).  Restricting to
     ``atom_index`` rows evaluates those rows only (the sum is per row).
     """
-    q = _as_f32_f64(q)
+    q = _as_f32_f64(q, cast_f32)
     f = _as_f32_f64(scatter)
     n, nq = f.shape
     qi_idx = np.arange(nq) if q_index is None else np.asarray(q_index)
@@ -240,13 +244,13 @@ def grad_fq(q, scatter, qbin, q_index=None, atom_index=None,
     return _nan_to_num_div(out, na)
 
 
-def grad_fq_nxn(q, scatter, qbin):
+def grad_fq_nxn(q, scatter, qbin, cast_f32=True):
     """-1/2 dF/dq by the N x N path (``cpu_nxn.py:137-197`` +
     ``nxn_cpu_wrap.py:269-334``): sum over j of a_ij (q_j - q_i) / na."""
     f = _as_f32_f64(scatter)
     n, nq = f.shape
     Q = q_values(qbin, nq)
-    d, r = pair_geometry_nxn(q)
+    d, r = pair_geometry_nxn(q, cast_f32)
     out = np.zeros((n, 3, nq))
     for i in range(n):
         for j in range(n):

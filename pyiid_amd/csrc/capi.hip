@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/pyiid_hip.h"
@@ -113,9 +114,11 @@ struct pyiid_ctx {
   // workspaces
   Buf gpart, fpart, forcepart, fq64, g64, io_a, io_b, io_c, small, u;
 
-  // timing
+  // timing: event pairs recorded around launches, resolved when queried
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<hipEvent_t> ev_pool;   // free events
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   double t_ms[4] = {0, 0, 0, 0};
   int64_t t_n[4] = {0, 0, 0, 0};
 };
@@ -230,20 +233,51 @@ int ensure_items(pyiid_ctx* c) {
   return PYIID_OK;
 }
 
+hipEvent_t take_event(pyiid_ctx* c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Resolve recorded event pairs into the per-kind totals.
+int resolve_timing(pyiid_ctx* c) {
+  for (auto& pe : c->ev_pending) {
+    HIPCHK(hipEventSynchronize(pe.second.second));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    c->t_ms[pe.first] += ms;
+    c->t_n[pe.first] += 1;
+    c->ev_pool.push_back(pe.second.first);
+    c->ev_pool.push_back(pe.second.second);
+  }
+  c->ev_pending.clear();
+  return PYIID_OK;
+}
+
+// Brackets one kernel launch with HIP events on the launch stream (no
+// synchronisation; the pair is resolved by pyiid_kernel_time).
 struct Timed {
   pyiid_ctx* c;
   int kind;
+  hipEvent_t e0 = nullptr;
   Timed(pyiid_ctx* c_, int k) : c(c_), kind(k) {
-    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+    if (c->timing) {
+      e0 = take_event(c);
+      if (e0) (void)hipEventRecord(e0, c->stream);
+    }
   }
   int done() {
-    if (!c->timing) return PYIID_OK;
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->t_ms[kind] += ms;
-    c->t_n[kind] += 1;
+    if (!c->timing || !e0) return PYIID_OK;
+    hipEvent_t e1 = take_event(c);
+    if (!e1) return fail(PYIID_E_HIP, "hipEventCreate failed");
+    HIPCHK(hipEventRecord(e1, c->stream));
+    c->ev_pending.push_back({kind, {e0, e1}});
+    if (c->ev_pending.size() > 4096) return resolve_timing(c);
     return PYIID_OK;
   }
 };
@@ -406,6 +440,11 @@ int pyiid_destroy(pyiid_ctx* c) {
     t.fso.release();
     t.gs.release();
   }
+  for (auto& pe : c->ev_pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -427,6 +466,7 @@ int pyiid_synchronize(pyiid_ctx* c) {
 
 int pyiid_enable_timing(pyiid_ctx* c, int enable) {
   CHK(check_ctx(c));
+  CHK(resolve_timing(c));
   c->timing = enable != 0;
   for (int k = 0; k < 4; ++k) {
     c->t_ms[k] = 0.0;
@@ -438,6 +478,7 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
 int pyiid_kernel_time(pyiid_ctx* c, int kind, double* total_ms, int64_t* launches) {
   CHK(check_ctx(c));
   if (kind < 0 || kind > 3) return fail(PYIID_E_INVALID, "kernel kind %d not in [0, 3]", kind);
+  CHK(resolve_timing(c));
   if (total_ms) *total_ms = c->t_ms[kind];
   if (launches) *launches = c->t_n[kind];
   return PYIID_OK;

@@ -89,8 +89,10 @@ __device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& 
 // reused as the transpose buffer of the end-of-kernel lane reduction.
 constexpr int kRedRow = 65;  // 64 lanes + 1 pad (bank spread)
 constexpr int kRedM = 16;    // Q bins reduced per pass
-constexpr int kSmemBytes = (kWaves * kRedM * kRedRow * 4 > kJTile * 32) ? kWaves * kRedM * kRedRow * 4
-                                                                      : kJTile * 32;
+constexpr int kStoreRow = kGradM + 4;  // 16-byte aligned rows for the store transpose
+constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+constexpr int kSmemBytes = cmax3(kWaves * kRedM * kRedRow * 4, kJTile * 32,
+                                 kWaves * 64 * kStoreRow * 4);
 
 // Sum acc[m] over the wave's 64 lanes in FP64 (through LDS); returns, in lane
 // L < M, the total of bin L.  Must be reached by every wave of the block.
@@ -113,6 +115,32 @@ __device__ __forceinline__ double lane_reduce_f64(const float (&acc)[M], unsigne
     }
   }
   return mine;
+}
+
+
+// Coalesced store of a wave's 64 (atoms) x M (bins) tile: lane l holds row l.
+// Rows go through LDS and leave as contiguous M-float row segments.
+// dst(row) = base + row * row_stride; rows >= n_rows and bins >= n_valid are
+// not written.  Must be reached by every wave of the block.
+template <int M>
+__device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char* smem, int wave,
+                                           int lane, float* base, size_t row_stride, int n_rows,
+                                           int n_valid) {
+  static_assert(64 % M == 0 && M <= kGradM, "tile width");
+  float* buf = reinterpret_cast<float*>(smem) + wave * 64 * kStoreRow;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < M; ++m) buf[lane * kStoreRow + m] = acc[m];
+  __syncthreads();
+  // dword stores: lanes sub = 0..M-1 of a row write M consecutive floats
+  // (the row stride is only 4-byte aligned in general).
+  constexpr int kRowsPer = 64 / M;  // rows per store instruction
+  const int sub = lane % M;
+#pragma unroll
+  for (int t = 0; t < 64 / kRowsPer; ++t) {
+    const int row = t * kRowsPer + lane / M;
+    if (row < n_rows && sub < n_valid) base[row * row_stride + sub] = buf[row * kStoreRow + sub];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -238,19 +266,15 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
       dst[2] = fz;
     }
   } else {
-    if (lane_ok && q0 < nq) {
-      float* gx = gpart + (((size_t)s * n + i) * 3) * nq + q0;
-      float* gy = gx + nq;
-      float* gz = gy + nq;
-#pragma unroll
-      for (int m = 0; m < M; ++m) {
-        if (q0 + m < nq) {
-          gx[m] = ax[m];
-          gy[m] = ay[m];
-          gz[m] = az[m];
-        }
-      }
-    }
+    // gpart[s][i][c][q0 + m]: rows of the tile are atoms, stride 3 nq floats.
+    const int i0 = blockIdx.x * 64;
+    const int n_rows = min(64, n - i0);
+    const int n_valid = max(0, min(M, nq - q0));
+    float* base = gpart + (((size_t)s * n + i0) * 3) * nq + (q0 < nq ? q0 : 0);
+    const size_t stride = (size_t)3 * nq;
+    store_tile<M>(ax, smem, wave, lane, base, stride, n_rows, n_valid);
+    store_tile<M>(ay, smem, wave, lane, base + nq, stride, n_rows, n_valid);
+    store_tile<M>(az, smem, wave, lane, base + 2 * (size_t)nq, stride, n_rows, n_valid);
   }
 }
 
