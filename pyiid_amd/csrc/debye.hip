@@ -10,8 +10,9 @@
 //    Each lane accumulates F and the three gradient components for its atom;
 //    no atomics, no K x Q intermediates.
 //  * fq_kernel: unordered pairs (j < i) for F(Q) alone; omega_m = sin(Q_m r)/r
-//    follows the three-term Chebyshev recurrence w_{m+1} = 2cos(dtheta) w_m -
-//    w_{m-1}, two VALU ops per pair and Q bin.
+//    follows the three-term recurrence w_{m+1} = (2 w_m - w_{m-1}) - eps w_m,
+//    eps = 4 sin^2(dtheta/2) from a half-angle sincos (accurate frequency for
+//    close pairs), three VALU ops per pair and Q bin.
 //  * partial sums leave each wave in FP64 and are reduced across tiles in
 //    FP64 by small deterministic reduction kernels.
 //
@@ -205,6 +206,8 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
       float sn, cs, s1, c1;
       sincos_red(q0d * dth, sn, cs);
       sincos_red(dth, s1, c1);
+      // Along Q the phasor (cs, sn) turns by dtheta: exact rotation (four
+      // FMA-class ops), accurate for every dtheta including close pairs.
       if constexpr (WEIGHTED) {
         const int ej = static_cast<int>(pj.w);
         const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
@@ -328,10 +331,14 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
       const AtomRec pj = tile[jj];
       const PairGeom g = pair_geom(pi, pj, lane_ok && (jt + jj < i));
       const double dth = dq * g.r;
-      float sn, cs, s1, c1;
+      float sn, cs, sh, ch;
       sincos_red(q0d * dth, sn, cs);
-      sincos_red(dth, s1, c1);
-      const float k = 2.0f * c1;
+      sincos_red(0.5 * dth, sh, ch);
+      // cos(dtheta) = 1 - 2 sin^2(dtheta/2) and eps = 2 - 2 cos(dtheta) = 4 sin^2(dtheta/2)
+      // keep their relative accuracy for small dtheta (close pairs).
+      const float s1 = 2.0f * sh * ch;
+      const float c1 = fmaf(-2.0f * sh, sh, 1.0f);
+      const float eps = 4.0f * sh * sh;
       float w0 = sn * g.inv_r;
       float w1 = fmaf(sn, c1, cs * s1) * g.inv_r;
       if constexpr (WEIGHTED) {
@@ -344,7 +351,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
           for (int e = 0; e < 4; ++e) {
             const int m = m4 * 4 + e;
             acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
-            const float w2 = fmaf(k, w1, -w0);
+            const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
             w0 = w1;
             w1 = w2;
           }
@@ -353,7 +360,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
         for (int m = 0; m < M; ++m) {
           acc[m] += w0;
-          const float w2 = fmaf(k, w1, -w0);
+          const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
           w0 = w1;
           w1 = w2;
         }
