@@ -78,13 +78,24 @@ int pyiid_set_positions(pyiid_ctx *ctx, const float *pos, int64_t n, int flags);
  * 'F(Q) scatter' / 'PDF scatter' that _wrap_atoms stores on the atoms
  * (elasticscatter/__init__.py:127-148) by an E x Q table plus this index. */
 int pyiid_set_elements(pyiid_ctx *ctx, const int32_t *elem, int64_t n, int flags);
+/* A batch of n_chain independent structures stored back to back: chain b is
+ * atoms [offsets[b], offsets[b+1]) of pos (float32 n_total x 3); offsets is a
+ * host array of n_chain + 1 entries from 0 to n_total.  The reference handles
+ * one structure per call (wrap_fq / wrap_fq_grad); batches serve the sampler
+ * (one F(Q), G(r), energy and force evaluation for every chain per launch).
+ * Elements reset to 0 when the layout changes (set them after this call). */
+int pyiid_set_batch(pyiid_ctx *ctx, const float *pos, int64_t n_total, const int64_t *offsets,
+                    int32_t n_chain, int flags);
+/* Current positions (float32 n_total x 3), e.g. after sampling. */
+int pyiid_get_positions(pyiid_ctx *ctx, float *pos, int flags);
 /* Per-element form-factor table f[e][q], float32 n_elem x n_q, for `slot`. */
 int pyiid_set_scatter(pyiid_ctx *ctx, int slot, const float *table, int32_t n_elem,
                       int32_t n_q, int flags);
 
 /* ---- Debye sums -------------------------------------------------------- */
 /* F(Q) on the full grid from Q=0: 2/na * sum_{i>j} f_i f_j sin(Q r)/r with
- * Q = m * float32(qbin), nan_to_num.  Writes n_q floats.
+ * Q = m * float32(qbin), nan_to_num.  Writes n_q floats per chain
+ * (n_chain x n_q).
  * Replaces wrap_fq(atoms, qbin, sum_type) of the CPU and GPU processors
  * (cpu_wrappers/flat_multi_cpu_wrap.py:22-60, gpu_wrappers/gpu_wrap.py:119-156). */
 int pyiid_fq(pyiid_ctx *ctx, int slot, double qbin, float *fq, int flags);
@@ -99,7 +110,7 @@ int pyiid_grad_fq(pyiid_ctx *ctx, int slot, double qbin, void *grad, float *fq, 
  * kernels/gpu_flat.py:11-49).  d or r may be NULL. */
 int pyiid_pair_geometry(pyiid_ctx *ctx, int64_t k_offset, int64_t k_count, float *d, float *r,
                         int flags);
-/* na(Q) = N * mean_{i>j} f_i f_j for `slot` (double, n_q).  Replaces
+/* na(Q) = N * mean_{i>j} f_i f_j for `slot` (double, n_chain x n_q).  Replaces
  * get_normalization_array + the mean in the wrappers
  * (cpu_flat.py:55-74, flat_multi_cpu_wrap.py:52-54). */
 int pyiid_normalization(pyiid_ctx *ctx, int slot, double *na, int flags);
@@ -130,12 +141,13 @@ int pyiid_potential_grad(pyiid_ctx *ctx, int potential, const double *grad_gcalc
                          const double *gobs, const double *gcalc, int32_t n, double *grad,
                          int flags);
 /* Calc1D energy and forces for a PDF target (calc_1d.py:78-95 with
- * exp_function=get_pdf, exp_grad_function=get_grad_pdf):
- * energy = conv * P(G), forces = conv * dP/dG . gradG in the reference's
- * convention (gradG built from the -1/2 dF/dq array).  G is computed from
- * the `slot` table at `qbin`, gobs has the transform's n_r points.  The
- * gradient array is never materialised: the adjoint u = T^T w is contracted
- * inside the Debye gradient kernel.  forces: N x 3 doubles (nullable). */
+ * exp_function=get_pdf, exp_grad_function=get_grad_pdf), for every chain:
+ * energy[b] = conv * P(G_b), forces = conv * dP/dG . gradG in the
+ * reference's convention (gradG built from the -1/2 dF/dq array).  G is
+ * computed from the `slot` table at `qbin`; gobs holds n_chain x n_r points.
+ * The gradient array is never materialised: the adjoint u = T^T w is
+ * contracted inside the Debye gradient kernel.  energy: n_chain doubles;
+ * forces: n_total x 3 doubles (nullable). */
 int pyiid_pdf_energy_forces(pyiid_ctx *ctx, int slot, double qbin, const double *gobs,
                             int potential, double conv, double *energy, double *forces,
                             int flags);
@@ -145,6 +157,24 @@ int pyiid_pdf_energy_forces(pyiid_ctx *ctx, int slot, double qbin, const double 
 int pyiid_fq_energy_forces(pyiid_ctx *ctx, int slot, double qbin, int32_t q_first,
                            const double *gobs, int potential, double conv, double *energy,
                            double *forces, int flags);
+
+/* ---- device-resident Hamiltonian Monte Carlo ---------------------------- */
+/* Prepare the sampler for every chain of the current structure against a
+ * PDF target (gobs: n_chain x n_r of the transform), potential and conv as
+ * in Calc1D (calc_1d.py:15-42), per-atom masses (n_total doubles).  Computes
+ * the initial energy and forces.  The sampler integrates exact forces
+ * -dE/dq, E = conv * P; the reference's Calc1D forces are half of that. */
+int pyiid_hmc_setup(pyiid_ctx *ctx, int slot, double qbin, const double *gobs, int potential,
+                    double conv, const double *masses, int flags);
+/* n_proposals HMC proposals per chain, each n_leapfrog velocity-Verlet steps
+ * of size `step` (pyiid/sim/__init__.py:10-38: p += eps/2 F, x += eps p/m,
+ * p += eps/2 F), momenta p ~ N(0, m/beta), accepted when
+ * u < exp(beta (H0 - H1)) (pyiid/sim/hmc.py:66-92, beta = its `temp`).
+ * Everything stays on the device; with use_graph the proposal is captured
+ * once as a HIP graph and replayed.  energies / accepted (nullable):
+ * n_proposals x n_chain, the energy after each decision and the decision. */
+int pyiid_hmc_run(pyiid_ctx *ctx, int n_proposals, int n_leapfrog, double step, double beta,
+                  uint64_t seed, int use_graph, double *energies, int32_t *accepted, int flags);
 
 #ifdef __cplusplus
 }

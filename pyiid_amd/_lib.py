@@ -61,6 +61,10 @@ _SIGNATURES = [
     ("pyiid_potential_grad", [_P, _I, _P, _I64, _P, _P, _I32, _P, _I]),
     ("pyiid_pdf_energy_forces", [_P, _I, _D, _P, _I, _D, ctypes.POINTER(_D), _P, _I]),
     ("pyiid_fq_energy_forces", [_P, _I, _D, _I32, _P, _I, _D, ctypes.POINTER(_D), _P, _I]),
+    ("pyiid_set_batch", [_P, _P, _I64, _P, _I32, _I]),
+    ("pyiid_get_positions", [_P, _P, _I]),
+    ("pyiid_hmc_setup", [_P, _I, _D, _P, _I, _D, _P, _I]),
+    ("pyiid_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _P, _P, _I]),
 ]
 EXPORTED = [name for name, _ in _SIGNATURES] + ["pyiid_last_error"]
 
@@ -161,6 +165,23 @@ class Context:
             n = pos.shape[0]
         check(self.lib.pyiid_set_positions(self.handle, _ptr(pos), n, flags))
 
+    def set_batch(self, positions_list):
+        """Load a batch of structures (list of (N_b, 3) arrays) back to back;
+        returns the offsets (B + 1)."""
+        sizes = [len(p) for p in positions_list]
+        offsets = np.zeros(len(sizes) + 1, np.int64)
+        offsets[1:] = np.cumsum(sizes)
+        pos = np.ascontiguousarray(np.concatenate([np.asarray(p, np.float32).reshape(-1, 3)
+                                                   for p in positions_list]), np.float32)
+        check(self.lib.pyiid_set_batch(self.handle, _ptr(pos), int(offsets[-1]), _ptr(offsets),
+                                       len(sizes), 0))
+        return offsets
+
+    def get_positions(self, n):
+        out = np.empty((n, 3), np.float32)
+        check(self.lib.pyiid_get_positions(self.handle, _ptr(out), 0))
+        return out
+
     def set_elements(self, elem, flags=0):
         if not flags & PYIID_DEVICE:
             elem = np.ascontiguousarray(elem, dtype=np.int32)
@@ -173,9 +194,9 @@ class Context:
         check(self.lib.pyiid_set_scatter(self.handle, slot, _ptr(table), e, q, flags))
 
     # --- Debye -------------------------------------------------------------
-    def fq(self, slot, qbin, nq, out=None, flags=0):
+    def fq(self, slot, qbin, nq, out=None, flags=0, n_chain=1):
         if out is None:
-            out = np.empty(nq, np.float32)
+            out = np.empty(nq if n_chain == 1 else (n_chain, nq), np.float32)
         check(self.lib.pyiid_fq(self.handle, slot, float(qbin), _ptr(out), flags))
         return out
 
@@ -195,8 +216,8 @@ class Context:
                                            _ptr(r), 0))
         return d, r
 
-    def normalization(self, slot, nq):
-        out = np.empty(nq, np.float64)
+    def normalization(self, slot, nq, n_chain=1):
+        out = np.empty(nq if n_chain == 1 else (n_chain, nq), np.float64)
         check(self.lib.pyiid_normalization(self.handle, slot, _ptr(out), 0))
         return out
 
@@ -248,13 +269,31 @@ class Context:
         return out
 
     def pdf_energy_forces(self, slot, qbin, gobs, potential, conv, n_atoms, forces=True):
+        """Energy and reference-convention forces.  ``gobs`` is (n_r,) for one
+        structure (returns a float energy) or (B, n_r) for a batch (returns B
+        energies)."""
         gobs = np.ascontiguousarray(gobs, dtype=np.float64)
-        e = ctypes.c_double()
+        n_chain = 1 if gobs.ndim == 1 else gobs.shape[0]
+        e = np.zeros(n_chain, np.float64)
         f = np.empty((n_atoms, 3), np.float64) if forces else None
-        check(self.lib.pyiid_pdf_energy_forces(self.handle, slot, float(qbin), _ptr(gobs),
-                                               potential, float(conv), ctypes.byref(e), _ptr(f),
-                                               0))
-        return e.value, f
+        check(self.lib.pyiid_pdf_energy_forces(
+            self.handle, slot, float(qbin), _ptr(gobs), potential, float(conv),
+            e.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(f), 0))
+        return (float(e[0]) if gobs.ndim == 1 else e), f
+
+    def hmc_setup(self, slot, qbin, gobs, potential, conv, masses):
+        gobs = np.ascontiguousarray(gobs, dtype=np.float64)
+        masses = np.ascontiguousarray(masses, dtype=np.float64)
+        check(self.lib.pyiid_hmc_setup(self.handle, slot, float(qbin), _ptr(gobs), potential,
+                                       float(conv), _ptr(masses), 0))
+
+    def hmc_run(self, n_proposals, n_leapfrog, step, beta, seed, n_chain, use_graph=True):
+        e = np.empty((n_proposals, n_chain), np.float64)
+        a = np.empty((n_proposals, n_chain), np.int32)
+        check(self.lib.pyiid_hmc_run(self.handle, int(n_proposals), int(n_leapfrog), float(step),
+                                     float(beta), int(seed) & (2 ** 64 - 1),
+                                     1 if use_graph else 0, _ptr(e), _ptr(a), 0))
+        return e, a
 
     def fq_energy_forces(self, slot, qbin, q_first, gobs, potential, conv, n_atoms, forces=True):
         gobs = np.ascontiguousarray(gobs, dtype=np.float64)

@@ -1,4 +1,9 @@
 // C ABI of the gfx950 scattering backend (declared in include/pyiid_hip.h).
+//
+// A context holds a batch of B independent structures ("chains") stored back
+// to back (B = 1 for the single-structure calls), the per-element form-factor
+// tables, the F(Q) -> G(r) transform, the work-item lists of the Debye
+// kernels, workspaces, and the device-resident sampler state.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,9 +39,9 @@ int fail(int code, const char* fmt, ...) {
                   __FILE__, __LINE__);                                                  \
   } while (0)
 
-#define CHK(expr)              \
-  do {                         \
-    int rc_ = (expr);          \
+#define CHK(expr)                    \
+  do {                               \
+    int rc_ = (expr);                \
     if (rc_ != PYIID_OK) return rc_; \
   } while (0)
 
@@ -75,9 +80,31 @@ struct Table {
   Buf tab;  // float [n_elem][nq]
   int n_elem = 0, nq = 0;
   bool set = false;
-  // derived normalisation (recomputed when the table or the composition changes)
-  Buf na, fsu, fso, gs;  // double [nq]
+  // per-chain normalisation [chain][nq] (recomputed when table or composition changes)
+  Buf na, fsu, fso, gs;
   uint64_t norm_stamp = ~0ull;
+};
+
+// A work-item list and its ranges, cached against the structure.
+struct ItemSet {
+  Buf items, ranges;
+  int n_items = 0, n_ranges = 0, split = 0;
+  uint64_t stamp = ~0ull;
+  int key = -1;
+};
+
+struct Hmc {
+  bool set = false;
+  int slot = 0, potential = 0;
+  double qbin = 0.0, conv = 1.0;
+  Buf mass, mom, fcur, fnew, pos_save, kin0, ecur, ehist, acchist, counter, chains, gobs;
+  int64_t hist_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_leap = -1;
+  double g_eps = 0.0, g_beta = 0.0;
+  uint64_t g_seed = 0, g_stamp = 0;
+  int64_t g_cap = 0;
 };
 
 }  // namespace
@@ -87,16 +114,19 @@ struct pyiid_ctx {
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
 
+  // structure: n_chain chains, atoms [off[b], off[b+1])
   int64_t n = 0;
+  int n_chain = 1;
+  std::vector<int64_t> off{0, 0};
   bool pos_set = false;
-  Buf posf;   // float [n][3] staging
-  Buf elem;   // int32 [n]
-  Buf pos;    // AtomRec [n]
+  Buf posf;  // float [n][3] staging
+  Buf elem;  // int32 [n]
+  Buf pos;   // AtomRec [n]
   bool pos_dirty = true;
-  std::vector<int64_t> counts{0};
+  std::vector<int32_t> elem_h;
   int max_elem = 0;
-  Buf counts_d;
-  uint64_t comp_stamp = 1;  // bumps when positions count / elements change
+  Buf counts_d, natoms_d;
+  uint64_t struct_stamp = 1;  // bumps when sizes / offsets / elements change
 
   Table tab[2];
   uint64_t tab_stamp[2] = {1, 1};
@@ -106,18 +136,18 @@ struct pyiid_ctx {
   int t_nr = 0, t_nq = 0;
   bool t_set = false;
 
-  // F-only work items
-  Buf items;
-  int n_items = 0;
-  int64_t items_n = -1;
+  ItemSet fq_items;     // unordered pairs, ranges per chain
+  ItemSet fused_items;  // ordered pairs with j splits, ranges per i tile
 
   // workspaces
-  Buf gpart, fpart, forcepart, fq64, g64, io_a, io_b, io_c, small, u;
+  Buf gpart, fpart, forcepart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, gobs;
+
+  Hmc hmc;
+  bool capturing = false;
 
   // timing: event pairs recorded around launches, resolved when queried
   bool timing = false;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  std::vector<hipEvent_t> ev_pool;   // free events
+  std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   double t_ms[4] = {0, 0, 0, 0};
   int64_t t_n[4] = {0, 0, 0, 0};
@@ -128,6 +158,11 @@ namespace {
 int ensure_device(pyiid_ctx* c) {
   HIPCHK(hipSetDevice(c->device));
   return PYIID_OK;
+}
+
+int check_ctx(pyiid_ctx* c) {
+  if (!c) return fail(PYIID_E_INVALID, "null context");
+  return ensure_device(c);
 }
 
 // Copy `bytes` from a user pointer (host or device per flags) into dst (device).
@@ -164,6 +199,14 @@ __global__ void pack_kernel(const float* __restrict__ p, const int32_t* __restri
   out[i] = a;
 }
 
+__global__ void unpack_kernel(const AtomRec* __restrict__ a, int64_t n, float* __restrict__ p) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[3 * i + 0] = static_cast<float>(a[i].x);
+  p[3 * i + 1] = static_cast<float>(a[i].y);
+  p[3 * i + 2] = static_cast<float>(a[i].z);
+}
+
 int ensure_packed(pyiid_ctx* c) {
   if (!c->pos_set) return fail(PYIID_E_STATE, "positions not set");
   if (!c->pos_dirty || c->n == 0) return PYIID_OK;
@@ -194,42 +237,119 @@ bool weighted_table(const Table& t) { return t.n_elem > 1; }
 
 int ensure_norm(pyiid_ctx* c, int slot) {
   Table& t = c->tab[slot];
-  const uint64_t stamp = c->tab_stamp[slot] * 1000003ull + c->comp_stamp;
+  const uint64_t stamp = c->tab_stamp[slot] * 1000003ull + c->struct_stamp;
   if (t.norm_stamp == stamp) return PYIID_OK;
-  const size_t b = sizeof(double) * std::max(1, t.nq);
+  if (c->capturing) return fail(PYIID_E_STATE, "normalisation changed during graph capture");
+  const int B = c->n_chain;
+  const size_t b = sizeof(double) * (size_t)std::max(1, t.nq) * B;
   CHK(t.na.reserve(b));
   CHK(t.fsu.reserve(b));
   CHK(t.fso.reserve(b));
   CHK(t.gs.reserve(b));
-  CHK(c->counts_d.reserve(sizeof(int64_t) * std::max<size_t>(1, t.n_elem)));
-  std::vector<int64_t> cnt(t.n_elem, 0);
-  for (size_t e = 0; e < c->counts.size() && e < cnt.size(); ++e) cnt[e] = c->counts[e];
-  HIPCHK(hipMemcpyAsync(c->counts_d.p, cnt.data(), sizeof(int64_t) * t.n_elem,
+  CHK(c->counts_d.reserve(sizeof(int64_t) * (size_t)std::max(1, t.n_elem) * B));
+  CHK(c->natoms_d.reserve(sizeof(int64_t) * B));
+  std::vector<int64_t> cnt((size_t)t.n_elem * B, 0), na(B);
+  for (int ch = 0; ch < B; ++ch) {
+    na[ch] = c->off[ch + 1] - c->off[ch];
+    for (int64_t i = c->off[ch]; i < c->off[ch + 1]; ++i) {
+      const int e = c->elem_h.empty() ? 0 : c->elem_h[i];
+      if (e < t.n_elem) cnt[(size_t)ch * t.n_elem + e]++;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(c->counts_d.p, cnt.data(), sizeof(int64_t) * cnt.size(),
                         hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->natoms_d.p, na.data(), sizeof(int64_t) * B, hipMemcpyHostToDevice,
+                        c->stream));
   HIPCHK(launch_norm(c->stream, t.tab.as<float>(), c->counts_d.as<int64_t>(), t.n_elem, t.nq,
-                     c->n, weighted_table(t), t.na.as<double>(), t.fsu.as<double>(),
-                     t.fso.as<double>(), t.gs.as<double>()));
-  // counts is a host temporary: wait for the copy before it goes out of scope
+                     c->natoms_d.as<int64_t>(), B, weighted_table(t), t.na.as<double>(),
+                     t.fsu.as<double>(), t.fso.as<double>(), t.gs.as<double>()));
+  // cnt / na are host temporaries: wait for the copies
   HIPCHK(hipStreamSynchronize(c->stream));
   t.norm_stamp = stamp;
   return PYIID_OK;
 }
 
-int ensure_items(pyiid_ctx* c) {
-  if (c->items_n == c->n) return PYIID_OK;
-  const int64_t n = c->n;
-  const int n_itile = static_cast<int>((n + 63) / 64);
-  int64_t jc = 256 * std::max<int64_t>(1, n / (256 * 16));
-  std::vector<FqItem> v;
-  for (int it = 0; it < n_itile; ++it) {
-    const int64_t jmax = std::min<int64_t>(n, (int64_t)it * 64 + 64);
-    for (int64_t jb = 0; jb < jmax; jb += jc)
-      v.push_back(FqItem{it, static_cast<int>(jb), static_cast<int>(std::min(jmax, jb + jc)), 0});
+int upload_items(pyiid_ctx* c, ItemSet& s, const std::vector<Item>& v, const std::vector<Range>& r) {
+  CHK(s.items.reserve(sizeof(Item) * std::max<size_t>(1, v.size())));
+  CHK(s.ranges.reserve(sizeof(Range) * std::max<size_t>(1, r.size())));
+  if (!v.empty())
+    HIPCHK(hipMemcpy(s.items.p, v.data(), sizeof(Item) * v.size(), hipMemcpyHostToDevice));
+  if (!r.empty())
+    HIPCHK(hipMemcpy(s.ranges.p, r.data(), sizeof(Range) * r.size(), hipMemcpyHostToDevice));
+  s.n_items = static_cast<int>(v.size());
+  s.n_ranges = static_cast<int>(r.size());
+  return PYIID_OK;
+}
+
+// Unordered-pair items: per chain, per 64-atom i tile, j ranges of the chain
+// below the tile's end.  Ranges: one per chain.
+int ensure_fq_items(pyiid_ctx* c) {
+  ItemSet& s = c->fq_items;
+  if (s.stamp == c->struct_stamp) return PYIID_OK;
+  if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  std::vector<Item> v;
+  std::vector<Range> r;
+  int tile = 0;
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int64_t o = c->off[b], nb = c->off[b + 1] - o;
+    const int begin = static_cast<int>(v.size());
+    const int64_t jc = 256 * std::max<int64_t>(1, nb / (256 * 16));
+    for (int64_t ib = o; ib < o + nb; ib += 64, ++tile) {
+      const int64_t ie = std::min(o + nb, ib + 64);
+      for (int64_t jb = o; jb < ie; jb += jc)
+        v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(jb),
+                         static_cast<int>(std::min(ie, jb + jc)), 0, b, tile, 0});
+    }
+    r.push_back(Range{begin, static_cast<int>(v.size()), static_cast<int>(o),
+                      static_cast<int>(o + nb)});
   }
-  CHK(c->items.reserve(sizeof(FqItem) * std::max<size_t>(1, v.size())));
-  HIPCHK(hipMemcpy(c->items.p, v.data(), sizeof(FqItem) * v.size(), hipMemcpyHostToDevice));
-  c->n_items = static_cast<int>(v.size());
-  c->items_n = n;
+  CHK(upload_items(c, s, v, r));
+  s.stamp = c->struct_stamp;
+  return PYIID_OK;
+}
+
+// Ordered-pair items: per chain, per i tile, `split` j ranges covering the
+// chain.  Every (tile, split) item exists (possibly with an empty j range) so
+// that the gradient partial buffer [split][n][3][nq] is fully written.
+// Ranges: one per i tile (for the force reduction).
+int ensure_fused_items(pyiid_ctx* c, int nq) {
+  ItemSet& s = c->fused_items;
+  int64_t n_tiles = 0, max_nb = 0;
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int64_t nb = c->off[b + 1] - c->off[b];
+    n_tiles += (nb + 63) / 64;
+    max_nb = std::max(max_nb, nb);
+  }
+  const int64_t base = std::max<int64_t>(1, n_tiles * ((nq + kGradM - 1) / kGradM));
+  int64_t split = (32768 + base - 1) / base;
+  split = std::min<int64_t>(split, 32);
+  split = std::min<int64_t>(split, std::max<int64_t>(1, max_nb / 64));
+  split = std::max<int64_t>(1, split);
+  if (s.stamp == c->struct_stamp && s.key == static_cast<int>(split)) return PYIID_OK;
+  if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  std::vector<Item> v;
+  std::vector<Range> r;
+  int tile = 0;
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int64_t o = c->off[b], nb = c->off[b + 1] - o;
+    const int64_t jc = (nb + split - 1) / split;
+    for (int64_t ib = o; ib < o + nb; ib += 64, ++tile) {
+      const int64_t ie = std::min(o + nb, ib + 64);
+      const int begin = static_cast<int>(v.size());
+      for (int64_t sp = 0; sp < split; ++sp) {
+        const int64_t jb = std::min(o + nb, o + sp * jc);
+        const int64_t je = std::min(o + nb, jb + jc);
+        v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(jb),
+                         static_cast<int>(je), static_cast<int>(sp), b, tile, 0});
+      }
+      r.push_back(Range{begin, static_cast<int>(v.size()), static_cast<int>(ib),
+                        static_cast<int>(ie)});
+    }
+  }
+  CHK(upload_items(c, s, v, r));
+  s.split = static_cast<int>(split);
+  s.key = static_cast<int>(split);
+  s.stamp = c->struct_stamp;
   return PYIID_OK;
 }
 
@@ -260,19 +380,20 @@ int resolve_timing(pyiid_ctx* c) {
 }
 
 // Brackets one kernel launch with HIP events on the launch stream (no
-// synchronisation; the pair is resolved by pyiid_kernel_time).
+// synchronisation; the pair is resolved by pyiid_kernel_time).  Inactive
+// while a graph is being captured.
 struct Timed {
   pyiid_ctx* c;
   int kind;
   hipEvent_t e0 = nullptr;
   Timed(pyiid_ctx* c_, int k) : c(c_), kind(k) {
-    if (c->timing) {
+    if (c->timing && !c->capturing) {
       e0 = take_event(c);
       if (e0) (void)hipEventRecord(e0, c->stream);
     }
   }
   int done() {
-    if (!c->timing || !e0) return PYIID_OK;
+    if (!e0) return PYIID_OK;
     hipEvent_t e1 = take_event(c);
     if (!e1) return fail(PYIID_E_HIP, "hipEventCreate failed");
     HIPCHK(hipEventRecord(e1, c->stream));
@@ -282,66 +403,57 @@ struct Timed {
   }
 };
 
-// Number of j splits of the fused kernel: enough waves to fill 256 CUs many
-// times over, each split covering at least 64 atoms.
-int choose_split(int64_t n, int nq) {
-  const int64_t base = ((n + 63) / 64) * ((nq + kGradM - 1) / kGradM);
-  int64_t s = (32768 + base - 1) / std::max<int64_t>(1, base);
-  s = std::min<int64_t>(s, 32);
-  s = std::min<int64_t>(s, std::max<int64_t>(1, n / 64));
-  return static_cast<int>(std::max<int64_t>(1, s));
+int64_t min_chain_atoms(pyiid_ctx* c) {
+  int64_t m = INT64_MAX;
+  for (int b = 0; b < c->n_chain; ++b) m = std::min(m, c->off[b + 1] - c->off[b]);
+  return c->n_chain ? m : 0;
 }
 
-// F(Q) (full grid) into c->fq64 (double, nq) using the F-only kernel.
+// F(Q) of every chain (full grid) into c->fq64 [B][nq] (double) and c->fq32.
 int compute_fq(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
-  const int nq = t.nq;
-  CHK(c->fq64.reserve(sizeof(double) * std::max(1, nq)));
-  if (c->n < 2) {
-    HIPCHK(hipMemsetAsync(c->fq64.p, 0, sizeof(double) * std::max(1, nq), c->stream));
-    return PYIID_OK;
-  }
+  const int nq = t.nq, B = c->n_chain;
+  CHK(c->fq64.reserve(sizeof(double) * (size_t)std::max(1, nq) * B));
+  CHK(c->fq32.reserve(sizeof(float) * (size_t)std::max(1, nq) * B));
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
-  CHK(ensure_items(c));
-  CHK(c->fpart.reserve(sizeof(double) * (size_t)c->n_items * nq));
+  CHK(ensure_fq_items(c));
+  CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, c->fq_items.n_items) * nq));
   const double dq = static_cast<double>(static_cast<float>(qbin));
   Timed tm(c, 1);
   HIPCHK(launch_fq(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n), t.tab.as<float>(),
-                   t.n_elem, nq, dq, c->items.as<FqItem>(), c->n_items, weighted_table(t),
-                   c->fpart.as<double>()));
+                   t.n_elem, nq, dq, c->fq_items.items.as<Item>(), c->fq_items.n_items,
+                   weighted_table(t), c->fpart.as<double>()));
   CHK(tm.done());
-  HIPCHK(launch_reduce_rows(c->stream, c->fpart.as<double>(), c->n_items, nq, t.fsu.as<double>(),
-                            c->fq64.as<double>(), nullptr));
+  // chains without pairs have empty ranges: their F is 0
+  HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), c->fq_items.ranges.as<Range>(), B,
+                              nq, t.fsu.as<double>(), c->fq64.as<double>(), c->fq32.as<float>()));
   return PYIID_OK;
 }
 
-// Fused kernel: gradient partials (or contracted forces when u != null) and F.
-int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, int* n_split_out,
-              int* n_part_force) {
+// Fused kernel over all chains: gradient partials (u == null) or contracted
+// force partials, plus F partials.
+int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u) {
   Table& t = c->tab[slot];
   const int nq = t.nq;
-  const int64_t n = c->n;
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
-  const int S = choose_split(n, nq);
-  const int j_chunk = static_cast<int>((n + S - 1) / S);
-  const int n_itile = static_cast<int>((n + 63) / 64);
-  const int n_qgroup = (nq + kWaves * kGradM - 1) / (kWaves * kGradM);
-  CHK(c->fpart.reserve(sizeof(double) * (size_t)S * n_itile * nq));
+  CHK(ensure_fused_items(c, nq));
+  const ItemSet& s = c->fused_items;
+  const int n_qchunk = ((nq + kWaves * kGradM - 1) / (kWaves * kGradM)) * kWaves;
+  CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, s.n_items) * nq));
   if (u) {
-    CHK(c->forcepart.reserve(sizeof(float) * (size_t)S * n_qgroup * kWaves * n * 3));
+    CHK(c->forcepart.reserve(sizeof(float) * (size_t)std::max(1, s.n_items) * n_qchunk * 64 * 3));
   } else {
-    CHK(c->gpart.reserve(sizeof(float) * (size_t)S * n * 3 * nq));
+    CHK(c->gpart.reserve(sizeof(float) * (size_t)s.split * std::max<int64_t>(1, c->n) * 3 * nq));
   }
   const double dq = static_cast<double>(static_cast<float>(qbin));
   Timed tm(c, 0);
-  HIPCHK(launch_fused_grad(c->stream, c->pos.as<AtomRec>(), static_cast<int>(n), t.tab.as<float>(),
-                           t.n_elem, nq, dq, S, j_chunk, weighted_table(t), c->gpart.as<float>(),
-                           c->fpart.as<double>(), u, c->forcepart.as<float>()));
+  HIPCHK(launch_fused_grad(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n),
+                           t.tab.as<float>(), t.n_elem, nq, dq, s.items.as<Item>(), s.n_items,
+                           weighted_table(t), c->gpart.as<float>(), c->fpart.as<double>(), u,
+                           c->forcepart.as<float>()));
   CHK(tm.done());
-  *n_split_out = S;
-  if (n_part_force) *n_part_force = S * n_qgroup * kWaves;
   return PYIID_OK;
 }
 
@@ -352,33 +464,105 @@ int gemm_t(pyiid_ctx* c, const void* a, bool a64, int64_t rows, double* out) {
   return tm.done();
 }
 
-int check_ctx(pyiid_ctx* c) {
-  if (!c) return fail(PYIID_E_INVALID, "null context");
-  return ensure_device(c);
-}
-
-// Energy + forces given gcalc (device, n_g doubles) and a u-builder.
-int potential_on_device(pyiid_ctx* c, int potential, const double* gobs_user, int n_g,
-                        const double* gcalc_dev, int flags, double* out5_host) {
-  CHK(c->io_b.reserve(sizeof(double) * n_g));  // gobs
-  CHK(c->io_c.reserve(sizeof(double) * n_g));  // w
-  CHK(c->small.reserve(sizeof(double) * 8));
-  CHK(upload(c, c->io_b.p, gobs_user, sizeof(double) * n_g, flags));
-  HIPCHK(launch_potential(c->stream, potential, c->io_b.as<double>(), gcalc_dev, n_g,
-                          c->small.as<double>(), c->io_c.as<double>()));
-  HIPCHK(hipMemcpyAsync(out5_host, c->small.p, sizeof(double) * 5, hipMemcpyDeviceToHost,
-                        c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+// Device-only energy and forces of every chain against gobs_dev [B][n_r]:
+// out5 [B][5], w [B][n_r]; forces (n x 3 doubles) = force_scale * dP/dG .
+// gradG in the reference convention (gradG from -1/2 dF/dq).  No host
+// synchronisation, no allocation once warm: safe inside graph capture.
+int energy_forces_device(pyiid_ctx* c, int slot, double qbin, int potential,
+                         const double* gobs_dev, double force_scale, double* forces_dev) {
+  const int nq = c->t_nq, nr = c->t_nr, B = c->n_chain;
+  CHK(compute_fq(c, slot, qbin));
+  CHK(c->g64.reserve(sizeof(double) * (size_t)nr * B));
+  CHK(gemm_t(c, c->fq32.p, false, B, c->g64.as<double>()));
+  CHK(c->out5.reserve(sizeof(double) * 5 * B));
+  CHK(c->w.reserve(sizeof(double) * (size_t)nr * B));
+  HIPCHK(launch_potential(c->stream, potential, gobs_dev, c->g64.as<double>(), nr, B,
+                          c->out5.as<double>(), c->w.as<double>()));
+  if (!forces_dev) return PYIID_OK;
+  CHK(c->u.reserve(sizeof(float) * (size_t)nq * B));
+  HIPCHK(launch_adjoint(c->stream, c->tmat.as<float>(), nr, nq, c->w.as<double>(),
+                        c->tab[slot].gs.as<double>(), B, c->u.as<float>()));
+  CHK(run_fused(c, slot, qbin, c->u.as<float>()));
+  const int n_qchunk = ((nq + kWaves * kGradM - 1) / (kWaves * kGradM)) * kWaves;
+  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
+                             c->fused_items.n_ranges, n_qchunk, force_scale, forces_dev));
   return PYIID_OK;
 }
 
-int forces_from_u(pyiid_ctx* c, int slot, double qbin, double conv, double* forces, int flags) {
-  int S = 0, n_part = 0;
-  CHK(run_fused(c, slot, qbin, c->u.as<float>(), &S, &n_part));
-  CHK(c->io_a.reserve(sizeof(double) * c->n * 3));
-  double* dst = (flags & PYIID_DEVICE) ? forces : c->io_a.as<double>();
-  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), n_part, c->n, conv, dst));
-  if (!(flags & PYIID_DEVICE)) CHK(download(c, forces, dst, sizeof(double) * c->n * 3, flags));
+int check_pdf_setup(pyiid_ctx* c, int slot, int potential) {
+  if (!c->pos_set) return fail(PYIID_E_STATE, "positions not set");
+  CHK(check_table(c, slot));
+  if (!c->t_set) return fail(PYIID_E_STATE, "transform not set");
+  if (c->tab[slot].nq != c->t_nq)
+    return fail(PYIID_E_INVALID, "table has %d Q bins, transform expects %d", c->tab[slot].nq,
+                c->t_nq);
+  if (potential != PYIID_POT_RW && potential != PYIID_POT_CHI_SQ)
+    return fail(PYIID_E_INVALID, "unknown potential %d", potential);
+  return PYIID_OK;
+}
+
+void destroy_graph(Hmc& h) {
+  if (h.exec) (void)hipGraphExecDestroy(h.exec);
+  if (h.graph) (void)hipGraphDestroy(h.graph);
+  h.exec = nullptr;
+  h.graph = nullptr;
+  h.g_leap = -1;
+}
+
+int set_structure(pyiid_ctx* c, const float* pos, int64_t n, const std::vector<int64_t>& off,
+                  int flags) {
+  if (n < 0 || n > 2000000000LL)
+    return fail(PYIID_E_INVALID, "atom count %lld out of range", (long long)n);
+  if (n > 0 && !pos) return fail(PYIID_E_INVALID, "null positions");
+  const bool same = (n == c->n) && (off == c->off);
+  if (!same) {
+    // the composition changes: elements reset to 0
+    CHK(c->elem.reserve(sizeof(int32_t) * std::max<int64_t>(1, n)));
+    c->elem_h.assign(static_cast<size_t>(n), 0);
+    if (n > 0)
+      HIPCHK(hipMemcpyAsync(c->elem.p, c->elem_h.data(), sizeof(int32_t) * n,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->max_elem = 0;
+    c->n = n;
+    c->off = off;
+    c->n_chain = static_cast<int>(off.size()) - 1;
+    c->struct_stamp++;
+    c->hmc.set = false;
+    destroy_graph(c->hmc);
+  }
+  CHK(c->posf.reserve(sizeof(float) * 3 * std::max<int64_t>(1, n)));
+  CHK(c->pos.reserve(sizeof(AtomRec) * std::max<int64_t>(1, n)));
+  CHK(upload(c, c->posf.p, pos, sizeof(float) * 3 * n, flags));
+  c->pos_set = true;
+  c->pos_dirty = true;
+  CHK(ensure_packed(c));
+  return finish(c, flags);
+}
+
+// One proposal of the batched sampler (fixed launch sequence).
+int hmc_proposal(pyiid_ctx* c, int n_leapfrog, double eps, double beta, uint64_t seed) {
+  Hmc& h = c->hmc;
+  const int B = c->n_chain;
+  const double fscale = 2.0 * h.conv;  // exact -dE/dq from the reference-convention contraction
+  HIPCHK(launch_hmc_begin(c->stream, c->pos.as<AtomRec>(), h.pos_save.as<double>(),
+                          h.mom.as<double>(), h.fcur.as<double>(), h.mass.as<double>(),
+                          h.chains.as<HmcChain>(), B, beta, eps, seed, h.counter.as<int64_t>(),
+                          h.kin0.as<double>()));
+  for (int l = 1; l <= n_leapfrog; ++l) {
+    HIPCHK(launch_hmc_drift(c->stream, c->pos.as<AtomRec>(), h.mom.as<double>(),
+                            h.mass.as<double>(), c->n, eps));
+    CHK(energy_forces_device(c, h.slot, h.qbin, h.potential, h.gobs.as<double>(), fscale,
+                             h.fnew.as<double>()));
+    HIPCHK(launch_hmc_kick(c->stream, h.mom.as<double>(), h.fnew.as<double>(), c->n,
+                           l < n_leapfrog ? eps : 0.5 * eps));
+  }
+  HIPCHK(launch_hmc_end(c->stream, c->pos.as<AtomRec>(), h.pos_save.as<double>(),
+                        h.mom.as<double>(), h.mass.as<double>(), h.fcur.as<double>(),
+                        h.fnew.as<double>(), h.ecur.as<double>(), c->out5.as<double>(), h.conv,
+                        h.chains.as<HmcChain>(), B, beta, seed, h.counter.as<int64_t>(),
+                        h.kin0.as<double>(), h.ehist.as<double>(), h.acchist.as<int32_t>(),
+                        h.hist_cap));
   return PYIID_OK;
 }
 
@@ -416,11 +600,6 @@ int pyiid_create(int device, pyiid_ctx** out) {
     return fail(PYIID_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
   }
   c->stream = c->own_stream;
-  if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-    (void)hipStreamDestroy(c->own_stream);
-    delete c;
-    return fail(PYIID_E_HIP, "hipEventCreate failed");
-  }
   *out = c;
   return PYIID_OK;
 }
@@ -429,9 +608,14 @@ int pyiid_destroy(pyiid_ctx* c) {
   if (!c) return PYIID_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (Buf* b : {&c->posf, &c->elem, &c->pos, &c->counts_d, &c->tmat, &c->items, &c->gpart,
-                 &c->fpart, &c->forcepart, &c->fq64, &c->g64, &c->io_a, &c->io_b, &c->io_c,
-                 &c->small, &c->u})
+  destroy_graph(c->hmc);
+  for (Buf* b : {&c->posf, &c->elem, &c->pos, &c->counts_d, &c->natoms_d, &c->tmat, &c->gpart,
+                 &c->fpart, &c->forcepart, &c->fq64, &c->fq32, &c->g64, &c->io_a, &c->io_b,
+                 &c->io_c, &c->w, &c->out5, &c->u, &c->gobs, &c->fq_items.items,
+                 &c->fq_items.ranges, &c->fused_items.items, &c->fused_items.ranges, &c->hmc.mass,
+                 &c->hmc.mom, &c->hmc.fcur, &c->hmc.fnew, &c->hmc.pos_save, &c->hmc.kin0,
+                 &c->hmc.ecur, &c->hmc.ehist, &c->hmc.acchist, &c->hmc.counter, &c->hmc.chains,
+                 &c->hmc.gobs})
     b->release();
   for (Table& t : c->tab) {
     t.tab.release();
@@ -445,8 +629,6 @@ int pyiid_destroy(pyiid_ctx* c) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return PYIID_OK;
@@ -455,6 +637,7 @@ int pyiid_destroy(pyiid_ctx* c) {
 int pyiid_set_stream(pyiid_ctx* c, void* stream) {
   CHK(check_ctx(c));
   c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+  destroy_graph(c->hmc);
   return PYIID_OK;
 }
 
@@ -486,30 +669,46 @@ int pyiid_kernel_time(pyiid_ctx* c, int kind, double* total_ms, int64_t* launche
 
 int pyiid_set_positions(pyiid_ctx* c, const float* pos, int64_t n, int flags) {
   CHK(check_ctx(c));
-  if (n < 0 || n > 2000000000LL) return fail(PYIID_E_INVALID, "atom count %lld out of range", (long long)n);
-  if (n > 0 && !pos) return fail(PYIID_E_INVALID, "null positions");
-  if (n != c->n) {
-    // composition changes with N: reset elements to all-zero
-    CHK(c->elem.reserve(sizeof(int32_t) * std::max<int64_t>(1, n)));
-    if (n > 0) HIPCHK(hipMemsetAsync(c->elem.p, 0, sizeof(int32_t) * n, c->stream));
-    c->counts.assign(1, n);
-    c->max_elem = 0;
-    c->n = n;
-    c->comp_stamp++;
-  }
-  CHK(c->posf.reserve(sizeof(float) * 3 * std::max<int64_t>(1, n)));
-  CHK(c->pos.reserve(sizeof(AtomRec) * std::max<int64_t>(1, n)));
-  CHK(upload(c, c->posf.p, pos, sizeof(float) * 3 * n, flags));
-  c->pos_set = true;
-  c->pos_dirty = true;
+  std::vector<int64_t> off{0, n};
+  return set_structure(c, pos, n, off, flags);
+}
+
+int pyiid_set_batch(pyiid_ctx* c, const float* pos, int64_t n_total, const int64_t* offsets,
+                    int32_t n_chain, int flags) {
+  CHK(check_ctx(c));
+  if (n_chain < 1 || !offsets) return fail(PYIID_E_INVALID, "need n_chain >= 1 and offsets");
+  std::vector<int64_t> off(offsets, offsets + n_chain + 1);
+  if (off[0] != 0 || off[n_chain] != n_total)
+    return fail(PYIID_E_INVALID, "offsets must run from 0 to n_total");
+  for (int b = 0; b < n_chain; ++b)
+    if (off[b + 1] < off[b]) return fail(PYIID_E_INVALID, "offsets must be non-decreasing");
+  return set_structure(c, pos, n_total, off, flags);
+}
+
+int pyiid_get_positions(pyiid_ctx* c, float* pos, int flags) {
+  CHK(check_ctx(c));
+  if (!c->pos_set) return fail(PYIID_E_STATE, "positions not set");
+  if (!pos) return fail(PYIID_E_INVALID, "null output");
   CHK(ensure_packed(c));
+  if (c->n == 0) return PYIID_OK;
+  float* dst = pos;
+  if (!(flags & PYIID_DEVICE)) {
+    CHK(c->io_a.reserve(sizeof(float) * 3 * c->n));
+    dst = c->io_a.as<float>();
+  }
+  hipLaunchKernelGGL(unpack_kernel, dim3(static_cast<unsigned>((c->n + 255) / 256)), dim3(256), 0,
+                     c->stream, c->pos.as<AtomRec>(), c->n, dst);
+  HIPCHK(hipGetLastError());
+  if (!(flags & PYIID_DEVICE)) CHK(download(c, pos, dst, sizeof(float) * 3 * c->n, flags));
   return finish(c, flags);
 }
 
 int pyiid_set_elements(pyiid_ctx* c, const int32_t* elem, int64_t n, int flags) {
   CHK(check_ctx(c));
   if (!c->pos_set) return fail(PYIID_E_STATE, "set positions before elements");
-  if (n != c->n) return fail(PYIID_E_INVALID, "element count %lld != atom count %lld", (long long)n, (long long)c->n);
+  if (n != c->n)
+    return fail(PYIID_E_INVALID, "element count %lld != atom count %lld", (long long)n,
+                (long long)c->n);
   std::vector<int32_t> h(static_cast<size_t>(n));
   if (n > 0) {
     if (!elem) return fail(PYIID_E_INVALID, "null elements");
@@ -525,13 +724,15 @@ int pyiid_set_elements(pyiid_ctx* c, const int32_t* elem, int64_t n, int flags) 
     if (e < 0 || e >= 1024) return fail(PYIID_E_INVALID, "element index %d out of range [0, 1024)", e);
     mx = std::max(mx, static_cast<int>(e));
   }
-  std::vector<int64_t> cnt(mx + 1, 0);
-  for (int32_t e : h) cnt[e]++;
-  c->counts = cnt;
+  c->elem_h = h;
   c->max_elem = mx;
-  if (n > 0) HIPCHK(hipMemcpyAsync(c->elem.p, h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(c->elem.p, h.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  c->comp_stamp++;
+  c->struct_stamp++;
+  c->hmc.set = false;
+  destroy_graph(c->hmc);
   c->pos_dirty = true;
   CHK(ensure_packed(c));
   return finish(c, flags);
@@ -543,7 +744,8 @@ int pyiid_set_scatter(pyiid_ctx* c, int slot, const float* table, int32_t n_elem
   if (slot != PYIID_SLOT_FQ && slot != PYIID_SLOT_PDF)
     return fail(PYIID_E_INVALID, "slot %d is not 0 or 1", slot);
   if (n_elem < 1 || n_q < 1 || !table)
-    return fail(PYIID_E_INVALID, "scatter table needs n_elem >= 1, n_q >= 1 (got %d, %d)", n_elem, n_q);
+    return fail(PYIID_E_INVALID, "scatter table needs n_elem >= 1, n_q >= 1 (got %d, %d)", n_elem,
+                n_q);
   Table& t = c->tab[slot];
   CHK(t.tab.reserve(sizeof(float) * (size_t)n_elem * n_q));
   CHK(upload(c, t.tab.p, table, sizeof(float) * (size_t)n_elem * n_q, flags));
@@ -551,6 +753,8 @@ int pyiid_set_scatter(pyiid_ctx* c, int slot, const float* table, int32_t n_elem
   t.nq = n_q;
   t.set = true;
   c->tab_stamp[slot]++;
+  c->hmc.set = false;
+  destroy_graph(c->hmc);
   return finish(c, flags);
 }
 
@@ -561,11 +765,13 @@ int pyiid_fq(pyiid_ctx* c, int slot, double qbin, float* fq, int flags) {
   if (!(qbin > 0.0)) return fail(PYIID_E_INVALID, "qbin must be positive");
   if (!fq) return fail(PYIID_E_INVALID, "null output");
   const int nq = c->tab[slot].nq;
+  const size_t cnt = (size_t)nq * c->n_chain;
   CHK(compute_fq(c, slot, qbin));
-  CHK(c->io_a.reserve(sizeof(float) * nq));
-  float* dst = (flags & PYIID_DEVICE) ? fq : c->io_a.as<float>();
-  HIPCHK(launch_f64_to_f32(c->stream, c->fq64.as<double>(), nq, dst));
-  if (!(flags & PYIID_DEVICE)) CHK(download(c, fq, dst, sizeof(float) * nq, flags));
+  if (flags & PYIID_DEVICE) {
+    HIPCHK(hipMemcpyAsync(fq, c->fq32.p, sizeof(float) * cnt, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    CHK(download(c, fq, c->fq32.p, sizeof(float) * cnt, flags));
+  }
   return finish(c, flags);
 }
 
@@ -575,6 +781,7 @@ int pyiid_grad_fq(pyiid_ctx* c, int slot, double qbin, void* grad, float* fq, in
   CHK(check_table(c, slot));
   if (!(qbin > 0.0)) return fail(PYIID_E_INVALID, "qbin must be positive");
   if (!grad) return fail(PYIID_E_INVALID, "null gradient output");
+  if (c->n_chain != 1) return fail(PYIID_E_INVALID, "pyiid_grad_fq takes a single structure");
   const int nq = c->tab[slot].nq;
   const int64_t n = c->n;
   const size_t esz = (flags & PYIID_F64) ? sizeof(double) : sizeof(float);
@@ -589,24 +796,28 @@ int pyiid_grad_fq(pyiid_ctx* c, int slot, double qbin, void* grad, float* fq, in
     }
     return finish(c, flags);
   }
-  int S = 0;
-  CHK(run_fused(c, slot, qbin, nullptr, &S, nullptr));
+  CHK(run_fused(c, slot, qbin, nullptr));
   Table& t = c->tab[slot];
   void* gdst = grad;
   if (!(flags & PYIID_DEVICE)) {
-    CHK(c->g64.reserve(gbytes));
-    gdst = c->g64.p;
+    CHK(c->io_c.reserve(gbytes));
+    gdst = c->io_c.p;
   }
-  HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), S, n, nq, t.gs.as<double>(), gdst,
-                            (flags & PYIID_F64) != 0));
+  HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, n, nq,
+                            t.gs.as<double>(), gdst, (flags & PYIID_F64) != 0));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, grad, gdst, gbytes, flags));
   if (fq) {
+    // F from all items of the single chain
+    const Range all{0, c->fused_items.n_items, 0, static_cast<int>(n)};
+    CHK(c->io_b.reserve(sizeof(Range)));
+    HIPCHK(hipMemcpyAsync(c->io_b.p, &all, sizeof(Range), hipMemcpyHostToDevice, c->stream));
     CHK(c->io_a.reserve(sizeof(float) * nq));
     float* fdst = (flags & PYIID_DEVICE) ? fq : c->io_a.as<float>();
-    const int n_itile = static_cast<int>((n + 63) / 64);
-    HIPCHK(launch_reduce_rows(c->stream, c->fpart.as<double>(), S * n_itile, nq,
-                              t.fso.as<double>(), nullptr, fdst));
+    HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), c->io_b.as<Range>(), 1, nq,
+                                t.fso.as<double>(), nullptr, fdst));
     if (!(flags & PYIID_DEVICE)) CHK(download(c, fq, fdst, sizeof(float) * nq, flags));
+    // `all` is a host temporary
+    HIPCHK(hipStreamSynchronize(c->stream));
   }
   return finish(c, flags);
 }
@@ -615,6 +826,7 @@ int pyiid_pair_geometry(pyiid_ctx* c, int64_t k_offset, int64_t k_count, float* 
                         int flags) {
   CHK(check_ctx(c));
   CHK(ensure_packed(c));
+  if (c->n_chain != 1) return fail(PYIID_E_INVALID, "pyiid_pair_geometry takes a single structure");
   const int64_t kmax = c->n * (c->n - 1) / 2;
   if (k_offset < 0 || k_count < 0 || k_offset + k_count > kmax)
     return fail(PYIID_E_INVALID, "pair range [%lld, %lld) outside [0, %lld)", (long long)k_offset,
@@ -623,9 +835,9 @@ int pyiid_pair_geometry(pyiid_ctx* c, int64_t k_offset, int64_t k_count, float* 
   float* dd = d;
   float* rr = r;
   if (!(flags & PYIID_DEVICE)) {
-    CHK(c->g64.reserve(sizeof(float) * 4 * (size_t)k_count));
-    dd = d ? c->g64.as<float>() : nullptr;
-    rr = r ? c->g64.as<float>() + 3 * k_count : nullptr;
+    CHK(c->io_c.reserve(sizeof(float) * 4 * (size_t)k_count));
+    dd = d ? c->io_c.as<float>() : nullptr;
+    rr = r ? c->io_c.as<float>() + 3 * k_count : nullptr;
   }
   Timed tm(c, 3);
   HIPCHK(launch_pair_geometry(c->stream, c->pos.as<AtomRec>(), k_offset, k_count, dd, rr));
@@ -643,7 +855,7 @@ int pyiid_normalization(pyiid_ctx* c, int slot, double* na, int flags) {
   CHK(check_table(c, slot));
   if (!na) return fail(PYIID_E_INVALID, "null output");
   CHK(ensure_norm(c, slot));
-  CHK(download(c, na, c->tab[slot].na.p, sizeof(double) * c->tab[slot].nq, flags));
+  CHK(download(c, na, c->tab[slot].na.p, sizeof(double) * c->tab[slot].nq * c->n_chain, flags));
   return finish(c, flags);
 }
 
@@ -659,7 +871,8 @@ int pyiid_set_transform(pyiid_ctx* c, double rstep, double qstep, const double* 
   const int64_t nfromdr = static_cast<int64_t>(std::ceil(M_PI / rstep / qstep));
   const int64_t length = std::max<int64_t>(n_q, nfromdr);
   const int64_t npad1 = static_cast<int64_t>(std::nearbyint(qmin / qstep)) + length;
-  const int64_t npad2 = (int64_t(1) << static_cast<int>(std::ceil(std::log(static_cast<double>(npad1)) / std::log(2.0)))) * 2;
+  const int64_t npad2 =
+      (int64_t(1) << static_cast<int>(std::ceil(std::log(static_cast<double>(npad1)) / std::log(2.0)))) * 2;
   const double drpad = M_PI / (static_cast<double>(npad2) * qstep);
   const int q_zero = static_cast<int>(std::ceil(qmin / qstep));
   std::vector<float> tm(static_cast<size_t>(n_r) * n_q);
@@ -685,6 +898,8 @@ int pyiid_set_transform(pyiid_ctx* c, double rstep, double qstep, const double* 
   c->t_nr = n_r;
   c->t_nq = n_q;
   c->t_set = true;
+  c->hmc.set = false;
+  destroy_graph(c->hmc);
   return PYIID_OK;
 }
 
@@ -700,8 +915,8 @@ int pyiid_pdf(pyiid_ctx* c, const float* fq, double* gr, int flags) {
   }
   double* dst = gr;
   if (!(flags & PYIID_DEVICE)) {
-    CHK(c->g64.reserve(sizeof(double) * c->t_nr));
-    dst = c->g64.as<double>();
+    CHK(c->io_c.reserve(sizeof(double) * c->t_nr));
+    dst = c->io_c.as<double>();
   }
   CHK(gemm_t(c, src, false, 1, dst));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, gr, dst, sizeof(double) * c->t_nr, flags));
@@ -722,8 +937,8 @@ int pyiid_grad_pdf(pyiid_ctx* c, const void* grad_fq, int64_t rows, double* grad
     CHK(c->io_a.reserve(in_bytes));
     CHK(upload(c, c->io_a.p, grad_fq, in_bytes, flags));
     src = c->io_a.p;
-    CHK(c->g64.reserve(out_bytes));
-    dst = c->g64.as<double>();
+    CHK(c->io_c.reserve(out_bytes));
+    dst = c->io_c.as<double>();
   }
   CHK(gemm_t(c, src, a64, rows, dst));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, grad_gr, dst, out_bytes, flags));
@@ -737,9 +952,16 @@ int pyiid_potential(pyiid_ctx* c, int potential, const double* gobs, const doubl
     return fail(PYIID_E_INVALID, "unknown potential %d", potential);
   if (n < 1 || !gobs || !gcalc) return fail(PYIID_E_INVALID, "bad arguments");
   CHK(c->io_a.reserve(sizeof(double) * n));
+  CHK(c->io_b.reserve(sizeof(double) * n));
+  CHK(c->w.reserve(sizeof(double) * n));
+  CHK(c->out5.reserve(sizeof(double) * 5));
   CHK(upload(c, c->io_a.p, gcalc, sizeof(double) * n, flags));
+  CHK(upload(c, c->io_b.p, gobs, sizeof(double) * n, flags));
+  HIPCHK(launch_potential(c->stream, potential, c->io_b.as<double>(), c->io_a.as<double>(), n, 1,
+                          c->out5.as<double>(), c->w.as<double>()));
   double out5[5];
-  CHK(potential_on_device(c, potential, gobs, n, c->io_a.as<double>(), flags, out5));
+  HIPCHK(hipMemcpyAsync(out5, c->out5.p, sizeof(out5), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (value) *value = out5[0];
   if (scale) *scale = out5[1];
   return PYIID_OK;
@@ -749,25 +971,20 @@ int pyiid_potential_grad(pyiid_ctx* c, int potential, const double* grad_gcalc, 
                          const double* gobs, const double* gcalc, int32_t n, double* grad,
                          int flags) {
   CHK(check_ctx(c));
-  if (potential != PYIID_POT_RW && potential != PYIID_POT_CHI_SQ)
-    return fail(PYIID_E_INVALID, "unknown potential %d", potential);
-  if (n < 1 || rows < 0 || !gobs || !gcalc || (rows > 0 && (!grad_gcalc || !grad)))
+  if (rows < 0 || (rows > 0 && (!grad_gcalc || !grad)))
     return fail(PYIID_E_INVALID, "bad arguments");
-  CHK(c->io_a.reserve(sizeof(double) * n));
-  CHK(upload(c, c->io_a.p, gcalc, sizeof(double) * n, flags));
-  double out5[5];
-  CHK(potential_on_device(c, potential, gobs, n, c->io_a.as<double>(), flags, out5));
+  CHK(pyiid_potential(c, potential, gobs, gcalc, n, nullptr, nullptr, flags));
   if (rows == 0) return PYIID_OK;
   const double* x = grad_gcalc;
   double* y = grad;
   if (!(flags & PYIID_DEVICE)) {
-    CHK(c->g64.reserve(sizeof(double) * (size_t)rows * n));
-    CHK(upload(c, c->g64.p, grad_gcalc, sizeof(double) * (size_t)rows * n, flags));
-    x = c->g64.as<double>();
+    CHK(c->io_c.reserve(sizeof(double) * (size_t)rows * n));
+    CHK(upload(c, c->io_c.p, grad_gcalc, sizeof(double) * (size_t)rows * n, flags));
+    x = c->io_c.as<double>();
     CHK(c->io_a.reserve(sizeof(double) * rows));
     y = c->io_a.as<double>();
   }
-  HIPCHK(launch_gemv_rows(c->stream, x, rows, n, c->io_c.as<double>(), y));
+  HIPCHK(launch_gemv_rows(c->stream, x, rows, n, c->w.as<double>(), y));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, grad, y, sizeof(double) * rows, flags));
   return finish(c, flags);
 }
@@ -775,34 +992,30 @@ int pyiid_potential_grad(pyiid_ctx* c, int potential, const double* grad_gcalc, 
 int pyiid_pdf_energy_forces(pyiid_ctx* c, int slot, double qbin, const double* gobs, int potential,
                             double conv, double* energy, double* forces, int flags) {
   CHK(check_ctx(c));
-  if (!c->pos_set) return fail(PYIID_E_STATE, "positions not set");
-  CHK(check_table(c, slot));
-  if (!c->t_set) return fail(PYIID_E_STATE, "transform not set");
-  if (c->tab[slot].nq != c->t_nq)
-    return fail(PYIID_E_INVALID, "table has %d Q bins, transform expects %d", c->tab[slot].nq, c->t_nq);
-  if (potential != PYIID_POT_RW && potential != PYIID_POT_CHI_SQ)
-    return fail(PYIID_E_INVALID, "unknown potential %d", potential);
+  CHK(check_pdf_setup(c, slot, potential));
   if (!gobs) return fail(PYIID_E_INVALID, "null target");
-  const int nq = c->t_nq, nr = c->t_nr;
-  CHK(compute_fq(c, slot, qbin));
-  CHK(c->io_a.reserve(sizeof(float) * nq));
-  HIPCHK(launch_f64_to_f32(c->stream, c->fq64.as<double>(), nq, c->io_a.as<float>()));
-  CHK(c->g64.reserve(sizeof(double) * nr));
-  CHK(gemm_t(c, c->io_a.p, false, 1, c->g64.as<double>()));
-  double out5[5];
-  CHK(potential_on_device(c, potential, gobs, nr, c->g64.as<double>(), flags, out5));
-  if (energy) *energy = out5[0] * conv;
-  if (!forces) return PYIID_OK;
-  if (c->n < 2) {
-    if (flags & PYIID_DEVICE) HIPCHK(hipMemsetAsync(forces, 0, sizeof(double) * 3 * c->n, c->stream));
-    else std::memset(forces, 0, sizeof(double) * 3 * c->n);
-    return finish(c, flags);
+  const int B = c->n_chain, nr = c->t_nr;
+  CHK(c->gobs.reserve(sizeof(double) * (size_t)nr * B));
+  CHK(upload(c, c->gobs.p, gobs, sizeof(double) * (size_t)nr * B, flags));
+  double* fdst = nullptr;
+  if (forces) {
+    if (flags & PYIID_DEVICE) {
+      fdst = forces;
+    } else {
+      CHK(c->io_c.reserve(sizeof(double) * 3 * std::max<int64_t>(1, c->n)));
+      fdst = c->io_c.as<double>();
+    }
   }
-  CHK(ensure_norm(c, slot));
-  CHK(c->u.reserve(sizeof(float) * nq));
-  HIPCHK(launch_adjoint(c->stream, c->tmat.as<float>(), nr, nq, c->io_c.as<double>(),
-                        c->tab[slot].gs.as<double>(), c->u.as<float>()));
-  CHK(forces_from_u(c, slot, qbin, conv, forces, flags));
+  CHK(energy_forces_device(c, slot, qbin, potential, c->gobs.as<double>(), conv, fdst));
+  if (energy) {
+    std::vector<double> o5(5 * (size_t)B);
+    HIPCHK(hipMemcpyAsync(o5.data(), c->out5.p, sizeof(double) * 5 * B, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < B; ++b) energy[b] = o5[5 * (size_t)b] * conv;
+  }
+  if (forces && !(flags & PYIID_DEVICE))
+    CHK(download(c, forces, fdst, sizeof(double) * 3 * c->n, flags));
   return finish(c, flags);
 }
 
@@ -812,6 +1025,7 @@ int pyiid_fq_energy_forces(pyiid_ctx* c, int slot, double qbin, int32_t q_first,
   CHK(check_ctx(c));
   if (!c->pos_set) return fail(PYIID_E_STATE, "positions not set");
   CHK(check_table(c, slot));
+  if (c->n_chain != 1) return fail(PYIID_E_INVALID, "pyiid_fq_energy_forces takes a single structure");
   if (potential != PYIID_POT_RW && potential != PYIID_POT_CHI_SQ)
     return fail(PYIID_E_INVALID, "unknown potential %d", potential);
   const int nq = c->tab[slot].nq;
@@ -819,8 +1033,16 @@ int pyiid_fq_energy_forces(pyiid_ctx* c, int slot, double qbin, int32_t q_first,
   if (!gobs) return fail(PYIID_E_INVALID, "null target");
   const int ng = nq - q_first;
   CHK(compute_fq(c, slot, qbin));
+  CHK(c->gobs.reserve(sizeof(double) * ng));
+  CHK(upload(c, c->gobs.p, gobs, sizeof(double) * ng, flags));
+  CHK(c->out5.reserve(sizeof(double) * 5));
+  CHK(c->w.reserve(sizeof(double) * ng));
+  HIPCHK(launch_potential(c->stream, potential, c->gobs.as<double>(),
+                          c->fq64.as<double>() + q_first, ng, 1, c->out5.as<double>(),
+                          c->w.as<double>()));
   double out5[5];
-  CHK(potential_on_device(c, potential, gobs, ng, c->fq64.as<double>() + q_first, flags, out5));
+  HIPCHK(hipMemcpyAsync(out5, c->out5.p, sizeof(out5), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (energy) *energy = out5[0] * conv;
   if (!forces) return PYIID_OK;
   if (c->n < 2) {
@@ -828,11 +1050,115 @@ int pyiid_fq_energy_forces(pyiid_ctx* c, int slot, double qbin, int32_t q_first,
     else std::memset(forces, 0, sizeof(double) * 3 * c->n);
     return finish(c, flags);
   }
-  CHK(ensure_norm(c, slot));
   CHK(c->u.reserve(sizeof(float) * nq));
-  HIPCHK(launch_adjoint_fq(c->stream, nq, q_first, c->io_c.as<double>(),
-                           c->tab[slot].gs.as<double>(), c->u.as<float>()));
-  CHK(forces_from_u(c, slot, qbin, conv, forces, flags));
+  HIPCHK(launch_adjoint_fq(c->stream, nq, q_first, c->w.as<double>(), c->tab[slot].gs.as<double>(),
+                           c->u.as<float>()));
+  CHK(run_fused(c, slot, qbin, c->u.as<float>()));
+  CHK(c->io_c.reserve(sizeof(double) * 3 * c->n));
+  double* fdst = (flags & PYIID_DEVICE) ? forces : c->io_c.as<double>();
+  const int n_qchunk = ((nq + kWaves * kGradM - 1) / (kWaves * kGradM)) * kWaves;
+  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
+                             c->fused_items.n_ranges, n_qchunk, conv, fdst));
+  if (!(flags & PYIID_DEVICE)) CHK(download(c, forces, fdst, sizeof(double) * 3 * c->n, flags));
+  return finish(c, flags);
+}
+
+int pyiid_hmc_setup(pyiid_ctx* c, int slot, double qbin, const double* gobs, int potential,
+                    double conv, const double* masses, int flags) {
+  CHK(check_ctx(c));
+  CHK(check_pdf_setup(c, slot, potential));
+  if (!gobs || !masses) return fail(PYIID_E_INVALID, "null target or masses");
+  if (min_chain_atoms(c) < 2) return fail(PYIID_E_INVALID, "every chain needs at least two atoms");
+  Hmc& h = c->hmc;
+  destroy_graph(h);
+  const int B = c->n_chain, nr = c->t_nr;
+  const int64_t n = c->n;
+  CHK(h.mass.reserve(sizeof(double) * n));
+  CHK(h.mom.reserve(sizeof(double) * 3 * n));
+  CHK(h.fcur.reserve(sizeof(double) * 3 * n));
+  CHK(h.fnew.reserve(sizeof(double) * 3 * n));
+  CHK(h.pos_save.reserve(sizeof(double) * 3 * n));
+  CHK(h.kin0.reserve(sizeof(double) * B));
+  CHK(h.ecur.reserve(sizeof(double) * B));
+  CHK(h.counter.reserve(sizeof(int64_t)));
+  CHK(h.chains.reserve(sizeof(HmcChain) * B));
+  CHK(h.gobs.reserve(sizeof(double) * (size_t)nr * B));
+  CHK(upload(c, h.mass.p, masses, sizeof(double) * n, flags));
+  CHK(upload(c, h.gobs.p, gobs, sizeof(double) * (size_t)nr * B, flags));
+  std::vector<HmcChain> ch(B);
+  for (int b = 0; b < B; ++b)
+    ch[b] = HmcChain{static_cast<int>(c->off[b]), static_cast<int>(c->off[b + 1])};
+  HIPCHK(hipMemcpyAsync(h.chains.p, ch.data(), sizeof(HmcChain) * B, hipMemcpyHostToDevice,
+                        c->stream));
+  h.slot = slot;
+  h.qbin = qbin;
+  h.potential = potential;
+  h.conv = conv;
+  // initial energy and exact forces of the current state
+  CHK(energy_forces_device(c, slot, qbin, potential, h.gobs.as<double>(), 2.0 * conv,
+                           h.fcur.as<double>()));
+  HIPCHK(launch_energy_from_out5(c->stream, c->out5.as<double>(), B, conv, h.ecur.as<double>()));
+  HIPCHK(hipStreamSynchronize(c->stream));  // `ch` is a host temporary
+  h.set = true;
+  return PYIID_OK;
+}
+
+int pyiid_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step, double beta,
+                  uint64_t seed, int use_graph, double* energies, int32_t* accepted, int flags) {
+  CHK(check_ctx(c));
+  Hmc& h = c->hmc;
+  if (!h.set) return fail(PYIID_E_STATE, "call pyiid_hmc_setup first");
+  if (n_proposals < 0 || n_leapfrog < 1 || !(beta > 0.0))
+    return fail(PYIID_E_INVALID, "need n_proposals >= 0, n_leapfrog >= 1, beta > 0");
+  const int B = c->n_chain;
+  if (n_proposals == 0) return PYIID_OK;
+  if (n_proposals > h.hist_cap) {
+    destroy_graph(h);  // the graph records the history buffers' addresses
+    CHK(h.ehist.reserve(sizeof(double) * (size_t)n_proposals * B));
+    CHK(h.acchist.reserve(sizeof(int32_t) * (size_t)n_proposals * B));
+    h.hist_cap = n_proposals;
+  }
+  const int64_t zero = 0;
+  HIPCHK(hipMemcpyAsync(h.counter.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int done = 0;
+  if (use_graph) {
+    const bool same = h.exec && h.g_leap == n_leapfrog && h.g_eps == step && h.g_beta == beta &&
+                      h.g_seed == seed && h.g_stamp == c->struct_stamp && h.g_cap == h.hist_cap;
+    if (!same) {
+      destroy_graph(h);
+      // one uncaptured proposal warms every buffer and cache, then capture
+      CHK(hmc_proposal(c, n_leapfrog, step, beta, seed));
+      done = 1;
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      c->capturing = true;
+      const int rc = hmc_proposal(c, n_leapfrog, step, beta, seed);
+      c->capturing = false;
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc != PYIID_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIPCHK(e);
+      h.graph = g;
+      HIPCHK(hipGraphInstantiate(&h.exec, h.graph, nullptr, nullptr, 0));
+      h.g_leap = n_leapfrog;
+      h.g_eps = step;
+      h.g_beta = beta;
+      h.g_seed = seed;
+      h.g_stamp = c->struct_stamp;
+      h.g_cap = h.hist_cap;
+    }
+    for (int p = done; p < n_proposals; ++p) HIPCHK(hipGraphLaunch(h.exec, c->stream));
+  } else {
+    for (int p = 0; p < n_proposals; ++p) CHK(hmc_proposal(c, n_leapfrog, step, beta, seed));
+  }
+  if (energies)
+    CHK(download(c, energies, h.ehist.p, sizeof(double) * (size_t)n_proposals * B, flags));
+  if (accepted)
+    CHK(download(c, accepted, h.acchist.p, sizeof(int32_t) * (size_t)n_proposals * B, flags));
   return finish(c, flags);
 }
 

@@ -146,31 +146,31 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 
 // ---------------------------------------------------------------------------
 // Fused F + gradient kernel (ordered pairs).
-//   grid: x = i tile (64 atoms), y = Q group (kWaves waves x M bins), z = j split
-//   gpart[s][i][c][q]   float  sum_j t_ij(q) (q_i - q_j)_c / r^2   (ff-weighted)
-//   fpart[s][itile][q]  double sum_{i in tile} sum_j ff sin(Q r)/r
-//   CONTRACT: instead of gpart, force_part[s * n_qchunk + qchunk][i][c] =
-//             sum_q u[q] * gpart(i, c, q)
+//   grid: x = work item (i tile x j range), y = Q group (kWaves waves x M bins)
+//   gpart[split][i][c][q]  float  sum_j t_ij(q) (q_i - q_j)_c / r^2 (ff-weighted)
+//   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
+//   CONTRACT: instead of gpart, force_part[item][qchunk][lane][c] =
+//             sum_q u[chain][q] * gpart(i, c, q)
 // ---------------------------------------------------------------------------
 template <int M, bool WEIGHTED, bool CONTRACT>
 __global__ __launch_bounds__(256) void fused_grad_kernel(
-    const AtomRec* __restrict__ pos, int n, const float* __restrict__ ftab, int n_elem, int nq,
-    double dq, int j_chunk, float* __restrict__ gpart, double* __restrict__ fpart,
-    const float* __restrict__ u, float* __restrict__ force_part) {
+    const AtomRec* __restrict__ pos, int n_total, const float* __restrict__ ftab, int n_elem,
+    int nq, double dq, const Item* __restrict__ items, float* __restrict__ gpart,
+    double* __restrict__ fpart, const float* __restrict__ u, float* __restrict__ force_part) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
+  const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = blockIdx.x * 64 + lane;
+  const int i = it.i_base + lane;
   const int qchunk = blockIdx.y * kWaves + wave;
   const int q0 = qchunk * M;
-  const int s = blockIdx.z;
-  const int j_begin = s * j_chunk;
-  const int j_end = min(n, j_begin + j_chunk);
-  const bool lane_ok = i < n;
-  const AtomRec pi = pos[lane_ok ? i : n - 1];
+  const int j_begin = it.jbeg;
+  const int j_end = it.jend;
+  const bool lane_ok = i < it.i_end;
+  const AtomRec pi = pos[lane_ok ? i : it.i_base];
 
   float qv[M];
 #pragma unroll
@@ -248,32 +248,29 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
 
   // F partial of this wave (FP64 lane reduction), lane m keeps bin q0 + m.
   const double fmine = lane_reduce_f64<M>(aF, smem, wave, lane);
-  const int n_itile = gridDim.x;
-  if (lane < M && q0 + lane < nq)
-    fpart[((size_t)s * n_itile + blockIdx.x) * nq + q0 + lane] = fmine;
+  if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = fmine;
 
   if constexpr (CONTRACT) {
+    const float* uc = u + (size_t)it.chain * nq;
     float fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const float um = (q0 + m < nq) ? u[q0 + m] : 0.0f;
+      const float um = (q0 + m < nq) ? uc[q0 + m] : 0.0f;
       fx = fmaf(um, ax[m], fx);
       fy = fmaf(um, ay[m], fy);
       fz = fmaf(um, az[m], fz);
     }
-    if (lane_ok) {
-      const size_t p = (size_t)s * (gridDim.y * kWaves) + qchunk;
-      float* dst = force_part + (p * n + i) * 3;
-      dst[0] = fx;
-      dst[1] = fy;
-      dst[2] = fz;
-    }
+    const size_t p = (size_t)blockIdx.x * (gridDim.y * kWaves) + qchunk;
+    float* dst = force_part + (p * 64 + lane) * 3;
+    dst[0] = fx;
+    dst[1] = fy;
+    dst[2] = fz;
   } else {
-    // gpart[s][i][c][q0 + m]: rows of the tile are atoms, stride 3 nq floats.
-    const int i0 = blockIdx.x * 64;
-    const int n_rows = min(64, n - i0);
+    // gpart[split][i][c][q0 + m]: rows of the tile are atoms, stride 3 nq floats.
+    const int i0 = it.i_base;
+    const int n_rows = it.i_end - it.i_base;
     const int n_valid = max(0, min(M, nq - q0));
-    float* base = gpart + (((size_t)s * n + i0) * 3) * nq + (q0 < nq ? q0 : 0);
+    float* base = gpart + (((size_t)it.split * n_total + i0) * 3) * nq + (q0 < nq ? q0 : 0);
     const size_t stride = (size_t)3 * nq;
     store_tile<M>(ax, smem, wave, lane, base, stride, n_rows, n_valid);
     store_tile<M>(ay, smem, wave, lane, base + nq, stride, n_rows, n_valid);
@@ -283,26 +280,25 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
 
 // ---------------------------------------------------------------------------
 // F-only kernel (unordered pairs j < i), Chebyshev recurrence on omega.
-//   grid: x = work item (i tile, j range), y = Q group
+//   grid: x = work item (i tile, j range of the same chain), y = Q group
 //   fpart[item][q] double
 // ---------------------------------------------------------------------------
 template <int M, bool WEIGHTED>
-__global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos, int n,
+__global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos,
                                                  const float* __restrict__ ftab, int n_elem,
-                                                 int nq, double dq,
-                                                 const FqItem* __restrict__ items,
+                                                 int nq, double dq, const Item* __restrict__ items,
                                                  double* __restrict__ fpart) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
-  const FqItem it = items[blockIdx.x];
+  const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int i = it.itile * 64 + lane;
+  const int i = it.i_base + lane;
   const int q0 = (blockIdx.y * kWaves + wave) * M;
-  const bool lane_ok = i < n;
-  const AtomRec pi = pos[lane_ok ? i : n - 1];
+  const bool lane_ok = i < it.i_end;
+  const AtomRec pi = pos[lane_ok ? i : it.i_base];
 
   float fi[WEIGHTED ? M : 1];
   if constexpr (WEIGHTED) {
@@ -372,16 +368,19 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
   if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine;
 }
 
-__global__ void reduce_rows_kernel(const double* __restrict__ part, int n_part, int nq,
-                                   const double* __restrict__ scale, double* __restrict__ out64,
-                                   float* __restrict__ out32) {
+__global__ void reduce_ranges_kernel(const double* __restrict__ part,
+                                     const Range* __restrict__ ranges, int n_rows, int nq,
+                                     const double* __restrict__ scale, double* __restrict__ out64,
+                                     float* __restrict__ out32) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
+  const int r = blockIdx.y;
+  if (q >= nq || r >= n_rows) return;
+  const Range rg = ranges[r];
   double acc = 0.0;
-  for (int p = 0; p < n_part; ++p) acc += part[(size_t)p * nq + q];
-  const double v = acc * scale[q];
-  if (out64) out64[q] = v;
-  if (out32) out32[q] = static_cast<float>(v);
+  for (int p = rg.begin; p < rg.end; ++p) acc += part[(size_t)p * nq + q];
+  const double v = acc * scale[(size_t)r * nq + q];
+  if (out64) out64[(size_t)r * nq + q] = v;
+  if (out32) out32[(size_t)r * nq + q] = static_cast<float>(v);
 }
 
 template <typename T>
@@ -395,28 +394,40 @@ __global__ void reduce_grad_kernel(const float* __restrict__ gpart, int n_split,
   }
 }
 
-__global__ void reduce_force_kernel(const float* __restrict__ fpart, int n_part, int64_t total,
-                                    double conv, double* __restrict__ out) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    double acc = 0.0;
-    for (int p = 0; p < n_part; ++p) acc += static_cast<double>(fpart[(size_t)p * total + e]);
-    out[e] = acc * conv;
-  }
+// force[i][c] = conv * sum over the tile's items and Q chunks (FP64); one
+// thread per (tile, lane, component).
+__global__ void reduce_force_kernel(const float* __restrict__ fpart, const Range* __restrict__ tiles,
+                                    int n_tiles, int n_qchunk, double conv,
+                                    double* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_tiles * 192) return;
+  const int tile = static_cast<int>(t / 192), rem = static_cast<int>(t % 192);
+  const int lane = rem / 3, c = rem % 3;
+  const Range tr = tiles[tile];
+  const int i = tr.i_base + lane;
+  if (i >= tr.i_end) return;
+  double acc = 0.0;
+  for (int p = tr.begin; p < tr.end; ++p)
+    for (int qc = 0; qc < n_qchunk; ++qc)
+      acc += static_cast<double>(fpart[(((size_t)p * n_qchunk + qc) * 64 + lane) * 3 + c]);
+  out[(size_t)i * 3 + c] = acc * conv;
 }
 
-// na(q) = N * mean_{i>j} f_i f_j = ((sum_e n_e f_e)^2 - sum_e n_e f_e^2) / (N - 1).
-// Unweighted kernels (one element) accumulate without f_i f_j = f^2, so their
-// scales carry w = f^2; weighted kernels carry w = 1.  0/0 -> 0 (nan_to_num).
+// na(q) = N * mean_{i>j} f_i f_j = ((sum_e n_e f_e)^2 - sum_e n_e f_e^2) / (N - 1),
+// per chain (counts[chain][e], n_atoms[chain]).  Unweighted kernels (one
+// element) accumulate without f_i f_j = f^2, so their scales carry w = f^2;
+// weighted kernels carry w = 1.  0/0 -> 0 (nan_to_num).
 __global__ void norm_kernel(const float* __restrict__ ftab, const int64_t* __restrict__ counts,
-                            int n_elem, int nq, int64_t n, int weighted, double* na,
-                            double* fsu, double* fso, double* gs) {
+                            int n_elem, int nq, const int64_t* __restrict__ n_atoms, int n_chain,
+                            int weighted, double* na, double* fsu, double* fso, double* gs) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
+  const int b = blockIdx.y;
+  if (q >= nq || b >= n_chain) return;
+  const int64_t n = n_atoms[b];
   double s1 = 0.0, s2 = 0.0;
   for (int e = 0; e < n_elem; ++e) {
     const double f = ftab[(size_t)e * nq + q];
-    const double c = static_cast<double>(counts[e]);
+    const double c = static_cast<double>(counts[(size_t)b * n_elem + e]);
     s1 += c * f;
     s2 += c * f * f;
   }
@@ -427,10 +438,11 @@ __global__ void norm_kernel(const float* __restrict__ ftab, const int64_t* __res
     w = weighted ? 1.0 : f0 * f0;
   }
   const bool ok = (n >= 2) && (nav != 0.0);
-  if (na) na[q] = (n >= 2) ? nav : __builtin_nan("");
-  fsu[q] = ok ? 2.0 * w / nav : 0.0;
-  fso[q] = ok ? w / nav : 0.0;
-  gs[q] = ok ? -w / nav : 0.0;
+  const size_t o = (size_t)b * nq + q;
+  if (na) na[o] = (n >= 2) ? nav : __builtin_nan("");
+  fsu[o] = ok ? 2.0 * w / nav : 0.0;
+  fso[o] = ok ? w / nav : 0.0;
+  gs[o] = ok ? -w / nav : 0.0;
 }
 
 // Flat pair k = j + i(i-1)/2 (i > j): d = q_i - q_j, r = |d| (float32 out).
@@ -465,50 +477,55 @@ int grid_for(int64_t total, int block) {
 
 }  // namespace
 
-hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n, const float* ftab,
-                             int n_elem, int nq, double dq, int n_split, int j_chunk,
+hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
+                             int n_elem, int nq, double dq, const Item* items, int n_items,
                              bool weighted, float* gpart, double* fpart, const float* u,
                              float* force_part) {
   constexpr int M = kGradM;
-  dim3 grid((n + 63) / 64, (nq + kWaves * M - 1) / (kWaves * M), n_split);
+  if (n_items <= 0) return hipSuccess;
+  dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
   dim3 block(256);
   const bool contract = (u != nullptr);
   if (weighted) {
     if (contract)
-      hipLaunchKernelGGL((fused_grad_kernel<M, true, true>), grid, block, 0, st, pos, n, ftab,
-                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, true, true>), grid, block, 0, st, pos, n_total,
+                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
     else
-      hipLaunchKernelGGL((fused_grad_kernel<M, true, false>), grid, block, 0, st, pos, n, ftab,
-                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, true, false>), grid, block, 0, st, pos, n_total,
+                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
   } else {
     if (contract)
-      hipLaunchKernelGGL((fused_grad_kernel<M, false, true>), grid, block, 0, st, pos, n, ftab,
-                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, false, true>), grid, block, 0, st, pos, n_total,
+                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
     else
-      hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, n, ftab,
-                         n_elem, nq, dq, j_chunk, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, n_total,
+                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n, const float* ftab, int n_elem,
-                     int nq, double dq, const FqItem* items, int n_items, bool weighted,
-                     double* fpart) {
+hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
+                     int n_elem, int nq, double dq, const Item* items, int n_items,
+                     bool weighted, double* fpart) {
   constexpr int M = kFqM;
+  (void)n_total;
+  if (n_items <= 0) return hipSuccess;
   dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
   if (weighted)
-    hipLaunchKernelGGL((fq_kernel<M, true>), grid, dim3(256), 0, st, pos, n, ftab, n_elem, nq,
-                       dq, items, fpart);
+    hipLaunchKernelGGL((fq_kernel<M, true>), grid, dim3(256), 0, st, pos, ftab, n_elem, nq, dq,
+                       items, fpart);
   else
-    hipLaunchKernelGGL((fq_kernel<M, false>), grid, dim3(256), 0, st, pos, n, ftab, n_elem, nq,
-                       dq, items, fpart);
+    hipLaunchKernelGGL((fq_kernel<M, false>), grid, dim3(256), 0, st, pos, ftab, n_elem, nq, dq,
+                       items, fpart);
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_rows(hipStream_t st, const double* part, int n_part, int nq,
-                              const double* scale, double* out_f64, float* out_f32) {
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, part, n_part,
-                     nq, scale, out_f64, out_f32);
+hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range* ranges,
+                                int n_rows, int nq, const double* scale, double* out_f64,
+                                float* out_f32) {
+  if (n_rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_ranges_kernel, dim3((nq + 255) / 256, n_rows), dim3(256), 0, st, part,
+                     ranges, n_rows, nq, scale, out_f64, out_f32);
   return hipGetLastError();
 }
 
@@ -524,19 +541,20 @@ hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, i
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_force(hipStream_t st, const float* fpart, int n_part, int64_t n,
-                               double conv, double* out) {
-  const int64_t total = n * 3;
-  hipLaunchKernelGGL(reduce_force_kernel, dim3(grid_for(total, 256)), dim3(256), 0, st, fpart,
-                     n_part, total, conv, out);
+hipError_t launch_reduce_force(hipStream_t st, const float* force_part, const Range* tiles,
+                               int n_tiles, int n_qchunk, double conv, double* out) {
+  const int64_t total = (int64_t)n_tiles * 192;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_force_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                     dim3(256), 0, st, force_part, tiles, n_tiles, n_qchunk, conv, out);
   return hipGetLastError();
 }
 
 hipError_t launch_norm(hipStream_t st, const float* ftab, const int64_t* counts, int n_elem,
-                       int nq, int64_t n, bool weighted, double* na, double* fsu, double* fso,
-                       double* gs) {
-  hipLaunchKernelGGL(norm_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, ftab, counts, n_elem,
-                     nq, n, weighted ? 1 : 0, na, fsu, fso, gs);
+                       int nq, const int64_t* n_atoms, int n_chain, bool weighted, double* na,
+                       double* fsu, double* fso, double* gs) {
+  hipLaunchKernelGGL(norm_kernel, dim3((nq + 255) / 256, n_chain), dim3(256), 0, st, ftab, counts,
+                     n_elem, nq, n_atoms, n_chain, weighted ? 1 : 0, na, fsu, fso, gs);
   return hipGetLastError();
 }
 

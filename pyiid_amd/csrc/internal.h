@@ -7,8 +7,9 @@
 
 namespace pyiid {
 
-// Atom record in HBM: float32 input positions widened exactly to double, and
-// the element index stored in w.  32 B per atom, one vector load.
+// Atom record in HBM: float32 input positions widened exactly to double (the
+// sampler integrates them in double), and the element index stored in w.
+// 32 B per atom, one vector load.
 struct alignas(32) AtomRec {
   double x, y, z, w;
 };
@@ -20,53 +21,95 @@ constexpr int kWaves = 4;      // waves per block in the Debye kernels
 constexpr int kJTile = 256;    // atoms staged in LDS per j tile
 constexpr int kMaxElem = 8;    // element-table rows staged in LDS (weighted kernels)
 
-// Work item of the F-only (unordered pair) kernel: i tile and j range.
-struct FqItem {
-  int itile, jbeg, jend, pad;
+// Work item of the Debye kernels.  Structures ("chains") are stored back to
+// back; an item is a 64-atom i tile [i_base, i_end) of one chain against the
+// j range [jbeg, jend) of the same chain.  `split` indexes the gradient
+// partial buffer, `tile` the chain-global i tile (for force reduction).
+struct Item {
+  int i_base, i_end, jbeg, jend;
+  int split, chain, tile, pad;
+};
+
+// Items [item_begin, item_end) of one i tile (or of one chain, for F).
+struct Range {
+  int begin, end, i_base, i_end;
 };
 
 // ---- launch wrappers (debye.hip) ----
-hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n, const float* ftab,
-                             int n_elem, int nq, double dq, int n_split, int j_chunk,
+// Ordered-pair F + gradient.  Non-contract: gpart[item.split][i][c][q] (float)
+// for i in the item's tile.  Contract (u != null): force_part[item][qchunk][64][3]
+// = sum_q u[chain][q] * grad(i, c, q).  fpart[item][q]: F partial (double).
+hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
+                             int n_elem, int nq, double dq, const Item* items, int n_items,
                              bool weighted, float* gpart, double* fpart, const float* u,
                              float* force_part);
-hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n, const float* ftab, int n_elem,
-                     int nq, double dq, const FqItem* items, int n_items, bool weighted,
-                     double* fpart);
-// out[q] = scale[q] * sum_p part[p * nq + q]   (FP64 cross-tile reduction)
-hipError_t launch_reduce_rows(hipStream_t st, const double* part, int n_part, int nq,
-                              const double* scale, double* out_f64, float* out_f32);
+// Unordered-pair F only: fpart[item][q] (double).
+hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
+                     int n_elem, int nq, double dq, const Item* items, int n_items,
+                     bool weighted, double* fpart);
+// out[r][q] = scale[r][q] * sum_{p in ranges[r]} part[p][q], r < n_rows (FP64)
+hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range* ranges,
+                                int n_rows, int nq, const double* scale, double* out_f64,
+                                float* out_f32);
 // grad[i][c][q] = gscale[q] * sum_s gpart[s][i][c][q]
 hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, int64_t n,
                               int nq, const double* gscale, void* out, bool out_f64);
-// force[i][c] = conv * sum_p fpart[p][i][c]
-hipError_t launch_reduce_force(hipStream_t st, const float* fpart, int n_part, int64_t n,
-                               double conv, double* out);
-// na[q], F scale 2w/na (unordered), F scale w/na (ordered), grad scale -w/na
+// force[i][c] = conv * sum over the tile's items and Q chunks of force_part
+hipError_t launch_reduce_force(hipStream_t st, const float* force_part, const Range* tiles,
+                               int n_tiles, int n_qchunk, double conv, double* out);
+// per-chain normalisation: na, 2w/na, w/na, -w/na  ([chain][q])
 hipError_t launch_norm(hipStream_t st, const float* ftab, const int64_t* counts, int n_elem,
-                       int nq, int64_t n, bool weighted, double* na, double* fscale_unordered,
-                       double* fscale_ordered, double* gscale);
+                       int nq, const int64_t* n_atoms, int n_chain, bool weighted, double* na,
+                       double* fscale_unordered, double* fscale_ordered, double* gscale);
 hipError_t launch_pair_geometry(hipStream_t st, const AtomRec* pos, int64_t k_offset,
                                 int64_t k_count, float* d, float* r);
 
 // ---- transform / potential (transform.hip) ----
-// C[m][ldc] (+)= A[m][K] . B[n][K]^T in FP32 MFMA; A float or double.
+// C[m][ldc] = A[m][K] . B[n][K]^T in FP32 MFMA; A float or double.
 hipError_t launch_gemm_nt(hipStream_t st, const void* a, bool a_f64, int64_t m, int k,
                           const float* b, int n, double* c_f64, float* c_f32);
-// potential value/scale and adjoint vector w (all FP64), single block.
-// out5 = {value, scale, a_opt, res_dot_res, g_dot_res}
+// Per chain b (one block each): potential value/scale and the adjoint
+// w[b][t] (FP64).  out5[b] = {value, scale, a_opt, res.res, g.res}.
 hipError_t launch_potential(hipStream_t st, int potential, const double* gobs,
-                            const double* gcalc, int n, double* out5, double* w);
+                            const double* gcalc, int n, int n_chain, double* out5, double* w);
 // y[row] = sum_t X[row][t] * w[t]  (FP64), one wave per row
 hipError_t launch_gemv_rows(hipStream_t st, const double* x, int64_t rows, int n,
                             const double* w, double* y);
-// u[q] = gscale[q] * sum_t T[t][q] * w[t]  for q in [0,nq); w has n_r entries
+// u[b][q] = gscale[b][q] * sum_t T[t][q] w[b][t]   (FP64 accumulation)
 hipError_t launch_adjoint(hipStream_t st, const float* t, int n_r, int nq, const double* w,
-                          const double* gscale, float* u);
+                          const double* gscale, int n_chain, float* u);
 // u[q] = gscale[q] * w[q - q_first] (0 below q_first)
 hipError_t launch_adjoint_fq(hipStream_t st, int nq, int q_first, const double* w,
                              const double* gscale, float* u);
 hipError_t launch_f64_to_f32(hipStream_t st, const double* x, int64_t n, float* y);
 hipError_t launch_f32_to_f64(hipStream_t st, const float* x, int64_t n, double* y);
+
+// ---- sampler (hmc.hip) ----
+struct HmcChain {
+  int atom_begin, atom_end;
+};
+// Start of a proposal: save positions, draw momenta p ~ N(0, m/beta) from the
+// counter-based generator (seed, proposal counter), kinetic energy K0[b],
+// first half kick p += eps/2 F.
+hipError_t launch_hmc_begin(hipStream_t st, AtomRec* pos, double* pos_save, double* mom,
+                            const double* force, const double* mass, const HmcChain* chains,
+                            int n_chain, double beta, double eps, uint64_t seed,
+                            const int64_t* counter, double* kin0);
+// x += eps p / m
+hipError_t launch_hmc_drift(hipStream_t st, AtomRec* pos, const double* mom, const double* mass,
+                            int64_t n, double eps);
+// p += h F
+hipError_t launch_hmc_kick(hipStream_t st, double* mom, const double* force, int64_t n, double h);
+// End of a proposal: K1, Metropolis with exp(beta (H0 - H1)), restore or
+// accept (positions, force, energy), record energy/acceptance, bump counter.
+hipError_t launch_hmc_end(hipStream_t st, AtomRec* pos, const double* pos_save, const double* mom,
+                          const double* mass, double* force_cur, const double* force_new,
+                          double* e_cur, const double* out5_new, double conv,
+                          const HmcChain* chains, int n_chain, double beta, uint64_t seed,
+                          int64_t* counter, const double* kin0, double* e_hist,
+                          int32_t* acc_hist, int64_t hist_cap);
+// e_cur[b] = conv * out5[b].value
+hipError_t launch_energy_from_out5(hipStream_t st, const double* out5, int n_chain, double conv,
+                                   double* e);
 
 }  // namespace pyiid

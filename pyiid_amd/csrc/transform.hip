@@ -97,17 +97,22 @@ __device__ double block_sum(double v, double* sh) {
   return t;
 }
 
-// Single block.  Rw: master_kernel.py:206-236; chi^2: :239-266; the adjoint
+// One block per chain.  Rw: master_kernel.py:206-236; chi^2: :239-266; the adjoint
 // w is d(potential)/d(gcalc) as get_grad_rw / get_grad_chi_sq apply it
 // (:293-375): grad = gradG . w with
 //   w = c * (s * res + (g.res)/(g.g) * (o - 2 a g)),
 //   c = -value/|res|^2 (Rw) or -2 (chi^2), res = o - s g, a = optimal scale.
 __global__ __launch_bounds__(1024) void potential_kernel(int potential,
-                                                         const double* __restrict__ o,
-                                                         const double* __restrict__ g, int n,
-                                                         double* __restrict__ out5,
-                                                         double* __restrict__ w) {
+                                                         const double* __restrict__ o_all,
+                                                         const double* __restrict__ g_all, int n,
+                                                         double* __restrict__ out5_all,
+                                                         double* __restrict__ w_all) {
   __shared__ double sh[16];
+  // one block per chain
+  const double* o = o_all + (size_t)blockIdx.x * n;
+  const double* g = g_all + (size_t)blockIdx.x * n;
+  double* out5 = out5_all + (size_t)blockIdx.x * 5;
+  double* w = w_all ? w_all + (size_t)blockIdx.x * n : nullptr;
   double go = 0.0, gg = 0.0, oo = 0.0;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     go += g[t] * o[t];
@@ -163,14 +168,17 @@ __global__ void gemv_rows_kernel(const double* __restrict__ x, int64_t rows, int
   if (lane == 0) y[row] = acc;
 }
 
+// u[b][q] = gscale[b][q] * sum_t T[t][q] w[b][t]  (FP64 accumulation; T float)
 __global__ void adjoint_kernel(const float* __restrict__ t, int n_r, int nq,
                                const double* __restrict__ w, const double* __restrict__ gscale,
                                float* __restrict__ u) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
   if (q >= nq) return;
+  const double* wb = w + (size_t)b * n_r;
   double acc = 0.0;
-  for (int r = 0; r < n_r; ++r) acc += static_cast<double>(t[(size_t)r * nq + q]) * w[r];
-  u[q] = static_cast<float>(acc * gscale[q]);
+  for (int r = 0; r < n_r; ++r) acc += static_cast<double>(t[(size_t)r * nq + q]) * wb[r];
+  u[(size_t)b * nq + q] = static_cast<float>(acc * gscale[(size_t)b * nq + q]);
 }
 
 __global__ void adjoint_fq_kernel(int nq, int q_first, const double* __restrict__ w,
@@ -214,9 +222,9 @@ hipError_t launch_gemm_nt(hipStream_t st, const void* a, bool a_f64, int64_t m, 
 }
 
 hipError_t launch_potential(hipStream_t st, int potential, const double* gobs,
-                            const double* gcalc, int n, double* out5, double* w) {
-  hipLaunchKernelGGL(potential_kernel, dim3(1), dim3(1024), 0, st, potential, gobs, gcalc, n, out5,
-                     w);
+                            const double* gcalc, int n, int n_chain, double* out5, double* w) {
+  hipLaunchKernelGGL(potential_kernel, dim3(n_chain), dim3(1024), 0, st, potential, gobs, gcalc, n,
+                     out5, w);
   return hipGetLastError();
 }
 
@@ -229,9 +237,9 @@ hipError_t launch_gemv_rows(hipStream_t st, const double* x, int64_t rows, int n
 }
 
 hipError_t launch_adjoint(hipStream_t st, const float* t, int n_r, int nq, const double* w,
-                          const double* gscale, float* u) {
-  hipLaunchKernelGGL(adjoint_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, t, n_r, nq, w,
-                     gscale, u);
+                          const double* gscale, int n_chain, float* u) {
+  hipLaunchKernelGGL(adjoint_kernel, dim3((nq + 255) / 256, n_chain), dim3(256), 0, st, t, n_r, nq,
+                     w, gscale, u);
   return hipGetLastError();
 }
 
