@@ -81,6 +81,15 @@ def load(path=LIB_PATH):
         if not os.path.exists(path):
             raise RuntimeError(
                 "libpyiid_hip.so is not built (%s); run __graft_entry__.build()" % path)
+        # PyTorch-ROCm wheels bundle their own libamdhip64 / libhsa-runtime64
+        # under the same sonames as /opt/rocm's.  Whichever is mapped first is
+        # the one every later library binds to; two HSA runtimes in one process
+        # leave torch without devices.  Map torch's first when it is present so
+        # this library and torch share one runtime (and device pointers).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(path)
         for name, argtypes in _SIGNATURES:
             fn = getattr(lib, name)
