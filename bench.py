@@ -81,12 +81,94 @@ def cpu_baseline(pos, table, index, target_s=15.0):
     }
 
 
+def _first_at_or_below(rw, target):
+    hit = np.nonzero(rw <= target)[0]
+    return int(hit[0]) if hit.size else None
+
+
+def hmc_comparison(n_exact, seed=0, ratio=3, train_every=10):
+    """Wall-seconds to a target Rw: exact-force HMC against the learned
+    integrator, 64 chains of 5,000-atom Au particles refined against the G(r)
+    of independently rattled copies (10 velocity-Verlet steps of 2e-3 per
+    proposal, unit masses, energy 100 Rw, beta 50).  The exact run's final
+    mean Rw + 0.005 is the target; both samplers start from the same
+    structures and are timed by HIP events from the first proposal on (the
+    surrogate's timeline includes its training steps)."""
+    from pyiid_amd import workloads
+    from pyiid_amd.elasticscatter import ElasticScatter
+    from pyiid_amd.sim import DeviceHMC, SurrogateHMC
+    conv, beta, eps, n_leap = 100.0, 50.0, 2e-3, 10
+    starts, targs, numbers = workloads.build("large", seed)
+    scat = ElasticScatter(dict(workloads.EXP), seed=0)
+    gt = workloads.target_pdfs(scat, targs, numbers)
+    structs = list(zip(starts, numbers))
+
+    ex = DeviceHMC(scat, structs, gt, conv=conv, masses="unit")
+    e, a, t = ex.run(n_exact, n_leapfrog=n_leap, step=eps, beta=beta, seed=seed, times=True)
+    ex.close()
+    rw_e = e.mean(axis=1) / conv
+    target = float(rw_e[-1]) + 0.005
+    k_e = _first_at_or_below(rw_e, target)
+
+    sur = SurrogateHMC(scat, structs, gt, conv=conv, masses="unit", net_seed=seed)
+    n_sur = ratio * n_exact
+    out = sur.run(n_sur, n_leapfrog=n_leap, step=eps, beta=beta, seed=seed,
+                  train_every=train_every, forget=0.5, ridge=1e-6, min_r2=0.5)
+    sur.close()
+    rw_s = out["energies"].mean(axis=1) / conv
+    ts = out["times_ms"]
+    k_s = _first_at_or_below(rw_s, target)
+    t_e = float(t[k_e]) / 1e3
+    t_s = float(ts[k_s]) / 1e3 if k_s is not None else None
+    return {
+        "workload": "64 Au chains x 5000 atoms (nearest FCC sites, a=4.08 A), start sigma=0.05 A, "
+                    "targets from sigma=0.1 A rattled copies; Q 0-25 step 0.1, r 0-40 step 0.01",
+        "leapfrog_steps": n_leap, "step": eps, "beta": beta, "conv": conv, "masses": "unit",
+        "target_rw": target,
+        "exact": {"proposals": n_exact, "s_per_proposal": float(t[-1]) / 1e3 / n_exact,
+                  "rw_start": float(rw_e[0]), "rw_final": float(rw_e[-1]),
+                  "acceptance": float(a.mean()), "wall_s_to_target": t_e,
+                  "wall_s_total": float(t[-1]) / 1e3},
+        "surrogate": {"proposals": n_sur, "train_every": train_every,
+                      "s_per_proposal": float(ts[-1]) / 1e3 / n_sur,
+                      "rw_final": float(rw_s[-1]), "acceptance": float(out["accepted"].mean()),
+                      "exact_fallback_proposals": int(out["exact_mode"].sum()),
+                      "guard_rejections": int(out["violations"].sum()),
+                      "wall_s_to_target": t_s},
+        "speedup_to_target": (t_e / t_s) if t_s else None,
+        "speedup_vs_exact_run": (float(t[-1]) / 1e3 / t_s) if t_s else None,
+    }
+
+
+def batched_hmc(n_prop=3, seed=0):
+    """Seconds per proposal for 256 uneven chains (100-3000 atoms), exact and
+    learned integrator, each proposal one captured HIP graph."""
+    from pyiid_amd import workloads
+    from pyiid_amd.elasticscatter import ElasticScatter
+    from pyiid_amd.sim import SurrogateHMC
+    starts, targs, numbers = workloads.build("batched", seed)
+    scat = ElasticScatter(dict(workloads.EXP), seed=0)
+    gt = workloads.target_pdfs(scat, targs, numbers)
+    h = SurrogateHMC(scat, list(zip(starts, numbers)), gt, conv=100.0, masses="unit")
+    kw = dict(n_leapfrog=10, step=2e-3, beta=50.0, seed=seed)
+    h.run_exact(1, **kw)                       # capture
+    _, _, t = h.run_exact(n_prop, times=True, **kw)
+    h.run(2, train_every=0, **kw)               # fit + capture
+    out = h.run(4 * n_prop, train_every=0, **kw)
+    h.close()
+    return {"chains": 256, "atoms": int(sum(len(s) for s in starts)),
+            "exact_s_per_proposal": float(t[-1]) / 1e3 / n_prop,
+            "surrogate_s_per_proposal": float(out["times_ms"][-1]) / 1e3 / (4 * n_prop)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--hmc-proposals", type=int, default=16,
+                    help="exact-HMC proposals of the sampler comparison (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -178,6 +260,9 @@ def main():
         },
     }
     ctx.close()
+    if args.hmc_proposals > 0:
+        out["hmc"] = hmc_comparison(args.hmc_proposals)
+        out["batched_hmc"] = batched_hmc()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(pos, table, index)
     print(json.dumps(out))

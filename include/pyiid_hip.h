@@ -172,9 +172,61 @@ int pyiid_hmc_setup(pyiid_ctx *ctx, int slot, double qbin, const double *gobs, i
  * u < exp(beta (H0 - H1)) (pyiid/sim/hmc.py:66-92, beta = its `temp`).
  * Everything stays on the device; with use_graph the proposal is captured
  * once as a HIP graph and replayed.  energies / accepted (nullable):
- * n_proposals x n_chain, the energy after each decision and the decision. */
+ * n_proposals x n_chain, the energy after each decision and the decision.
+ * times_ms (nullable, host): completion time of each proposal in ms since
+ * the call started, from HIP events on the context stream. */
 int pyiid_hmc_run(pyiid_ctx *ctx, int n_proposals, int n_leapfrog, double step, double beta,
-                  uint64_t seed, int use_graph, double *energies, int32_t *accepted, int flags);
+                  uint64_t seed, int use_graph, double *energies, int32_t *accepted,
+                  double *times_ms, int flags);
+
+/* ---- learned HMC integrator --------------------------------------------- */
+/* No counterpart in the reference: pyiid/sim/hmc.py:33-63 integrates with the
+ * exact forces only.  Here a per-atom network drives the leapfrog and the
+ * exact Rw (pyiid_hmc_setup's potential) still decides every Metropolis step
+ * (hmc.py:66-92), so the sampled distribution is unchanged.
+ *
+ * Descriptors: D[i][m] = sum_{j != i, r_ij < rc} exp(-eta (r_ij - mu_m)^2) *
+ * (cos(pi r_ij / rc) + 1) / 2, m < 16, mu_m = mu0 + m * dmu, over the atoms of
+ * i's chain.  Network per atom: x = (D - d_mean) * d_inv_scale, 64 SiLU units,
+ * 64 SiLU units, one output; weights in BF16, FP32 accumulation.  Surrogate
+ * energy of a chain: sum over its atoms.  params = {rc, skin, mu0, dmu, eta}
+ * (A, 1/A^2); neighbour lists hold pairs within rc + skin of a proposal's
+ * start, and a proposal that moves an atom farther than skin / 2 is rejected.
+ * w1: 16 x 64, b1: 64, w2: 64 x 64, b2: 64, d_mean / d_inv_scale: 16 (host
+ * float32, rounded to BF16 here).  max_nbr: neighbour-list capacity per atom.
+ * Resets the output-layer fit. */
+int pyiid_sur_setup(pyiid_ctx *ctx, const float *params, const float *w1, const float *b1,
+                    const float *w2, const float *b2, const float *d_mean,
+                    const float *d_inv_scale, int32_t max_nbr);
+/* Output weights w3 (n_chain x 64 floats; rounded to BF16), one row per chain. */
+int pyiid_sur_set_output(pyiid_ctx *ctx, const float *w3, int flags);
+/* Descriptors of the current positions (n_total x 16 floats). */
+int pyiid_sur_descriptors(pyiid_ctx *ctx, float *d, int flags);
+/* Surrogate forces -dE_s/dq (n_total x 3 doubles, nullable) and per-atom
+ * network outputs (n_total floats, nullable) at the current positions. */
+int pyiid_sur_forces(pyiid_ctx *ctx, double *forces, float *atom_energy, int flags);
+/* Feature forces: row k (k < 64) holds the surrogate forces with w3 = e_k
+ * for every chain, 64 x (3 n_total) floats. */
+int pyiid_sur_features(pyiid_ctx *ctx, float *phi, int flags);
+/* One training step at the current positions: exact energies and forces
+ * (Debye path, as in pyiid_hmc_setup), the normal equations of
+ * min |Phi_b w - F_b|^2 added to each chain's running sums after scaling those
+ * by `forget`, ridge solve (lambda = ridge * trace / 64) for w3.  r2 (nullable,
+ * n_chain doubles): 1 - |F - Phi w|^2 / |F|^2 on this sample. */
+int pyiid_sur_train(pyiid_ctx *ctx, double forget, double ridge, double *r2, int flags);
+/* n_proposals proposals driven by the surrogate: forces from the network,
+ * velocity Verlet and momenta as in pyiid_hmc_run, acceptance with the exact
+ * energy of the end point.  A training step runs before proposal p whenever
+ * p % train_every == 0 (train_every = 0: only if no fit exists).  When the
+ * mean r2 of a training step is below min_r2, the proposals until the next
+ * step use exact forces (pyiid_hmc_run's proposal).  energies / accepted /
+ * exact_mode (per proposal) / times_ms as in pyiid_hmc_run; violations
+ * (n_chain int32, nullable): proposals rejected for leaving the neighbour
+ * lists' range, counted since the layout was set. */
+int pyiid_sur_hmc_run(pyiid_ctx *ctx, int n_proposals, int n_leapfrog, double step, double beta,
+                      uint64_t seed, int use_graph, int train_every, double forget, double ridge,
+                      double min_r2, double *energies, int32_t *accepted, int32_t *exact_mode,
+                      double *times_ms, int32_t *violations, int flags);
 
 #ifdef __cplusplus
 }

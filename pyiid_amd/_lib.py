@@ -64,7 +64,15 @@ _SIGNATURES = [
     ("pyiid_set_batch", [_P, _P, _I64, _P, _I32, _I]),
     ("pyiid_get_positions", [_P, _P, _I]),
     ("pyiid_hmc_setup", [_P, _I, _D, _P, _I, _D, _P, _I]),
-    ("pyiid_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _P, _P, _I]),
+    ("pyiid_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _P, _P, _P, _I]),
+    ("pyiid_sur_setup", [_P, _P, _P, _P, _P, _P, _P, _P, _I32]),
+    ("pyiid_sur_set_output", [_P, _P, _I]),
+    ("pyiid_sur_descriptors", [_P, _P, _I]),
+    ("pyiid_sur_forces", [_P, _P, _P, _I]),
+    ("pyiid_sur_features", [_P, _P, _I]),
+    ("pyiid_sur_train", [_P, _D, _D, _P, _I]),
+    ("pyiid_sur_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _I, _D, _D, _D, _P, _P, _P,
+                           _P, _P, _I]),
 ]
 EXPORTED = [name for name, _ in _SIGNATURES] + ["pyiid_last_error"]
 
@@ -296,13 +304,63 @@ class Context:
         check(self.lib.pyiid_hmc_setup(self.handle, slot, float(qbin), _ptr(gobs), potential,
                                        float(conv), _ptr(masses), 0))
 
-    def hmc_run(self, n_proposals, n_leapfrog, step, beta, seed, n_chain, use_graph=True):
+    def hmc_run(self, n_proposals, n_leapfrog, step, beta, seed, n_chain, use_graph=True,
+                times=False):
         e = np.empty((n_proposals, n_chain), np.float64)
         a = np.empty((n_proposals, n_chain), np.int32)
+        t = np.empty(n_proposals, np.float64) if times else None
         check(self.lib.pyiid_hmc_run(self.handle, int(n_proposals), int(n_leapfrog), float(step),
                                      float(beta), int(seed) & (2 ** 64 - 1),
-                                     1 if use_graph else 0, _ptr(e), _ptr(a), 0))
-        return e, a
+                                     1 if use_graph else 0, _ptr(e), _ptr(a), _ptr(t), 0))
+        return (e, a, t) if times else (e, a)
+
+    # ---- learned integrator ----
+    def sur_setup(self, params, w1, b1, w2, b2, d_mean, d_inv_scale, max_nbr):
+        f = [np.ascontiguousarray(x, dtype=np.float32) for x in
+             (params, w1, b1, w2, b2, d_mean, d_inv_scale)]
+        if f[0].size != 5 or f[1].shape != (16, 64) or f[2].size != 64 or f[3].shape != (64, 64) \
+                or f[4].size != 64 or f[5].size != 16 or f[6].size != 16:
+            raise ValueError("surrogate parameter shapes: params 5, w1 16x64, b1 64, w2 64x64, "
+                             "b2 64, d_mean 16, d_inv_scale 16")
+        check(self.lib.pyiid_sur_setup(self.handle, *[_ptr(x) for x in f], int(max_nbr)))
+
+    def sur_set_output(self, w3):
+        w3 = np.ascontiguousarray(w3, dtype=np.float32)
+        check(self.lib.pyiid_sur_set_output(self.handle, _ptr(w3), 0))
+
+    def sur_descriptors(self, n):
+        d = np.empty((n, 16), np.float32)
+        check(self.lib.pyiid_sur_descriptors(self.handle, _ptr(d), 0))
+        return d
+
+    def sur_forces(self, n, energies=False):
+        f = np.empty((n, 3), np.float64)
+        e = np.empty(n, np.float32) if energies else None
+        check(self.lib.pyiid_sur_forces(self.handle, _ptr(f), _ptr(e), 0))
+        return (f, e) if energies else f
+
+    def sur_features(self, n):
+        phi = np.empty((64, 3 * n), np.float32)
+        check(self.lib.pyiid_sur_features(self.handle, _ptr(phi), 0))
+        return phi
+
+    def sur_train(self, n_chain, forget=0.0, ridge=1e-6):
+        r2 = np.empty(n_chain, np.float64)
+        check(self.lib.pyiid_sur_train(self.handle, float(forget), float(ridge), _ptr(r2), 0))
+        return r2
+
+    def sur_hmc_run(self, n_proposals, n_leapfrog, step, beta, seed, n_chain, use_graph=True,
+                    train_every=0, forget=0.5, ridge=1e-6, min_r2=-np.inf):
+        e = np.empty((n_proposals, n_chain), np.float64)
+        a = np.empty((n_proposals, n_chain), np.int32)
+        m = np.empty(n_proposals, np.int32)
+        t = np.empty(n_proposals, np.float64)
+        v = np.empty(n_chain, np.int32)
+        check(self.lib.pyiid_sur_hmc_run(
+            self.handle, int(n_proposals), int(n_leapfrog), float(step), float(beta),
+            int(seed) & (2 ** 64 - 1), 1 if use_graph else 0, int(train_every), float(forget),
+            float(ridge), float(min_r2), _ptr(e), _ptr(a), _ptr(m), _ptr(t), _ptr(v), 0))
+        return {"energies": e, "accepted": a, "exact_mode": m, "times_ms": t, "violations": v}
 
     def fq_energy_forces(self, slot, qbin, q_first, gobs, potential, conv, n_atoms, forces=True):
         gobs = np.ascontiguousarray(gobs, dtype=np.float64)

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -45,12 +47,17 @@ int fail(int code, const char* fmt, ...) {
     if (rc_ != PYIID_OK) return rc_; \
   } while (0)
 
+// Counts device reallocations; a captured graph records it and is captured
+// again when any workspace it may reference has moved since.
+std::atomic<uint64_t> g_alloc_epoch{1};
+
 // Device buffer that only grows.
 struct Buf {
   void* p = nullptr;
   size_t bytes = 0;
   int reserve(size_t want) {
     if (want <= bytes) return PYIID_OK;
+    g_alloc_epoch.fetch_add(1);
     if (p) {
       (void)hipFree(p);
       p = nullptr;
@@ -93,18 +100,38 @@ struct ItemSet {
   int key = -1;
 };
 
+// One captured proposal and the parameters it was captured with.
+struct GraphSlot {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int leap = -1;
+  double eps = 0.0, beta = 0.0;
+  uint64_t seed = 0, stamp = 0, epoch = 0;
+  int64_t cap = 0;
+};
+
 struct Hmc {
   bool set = false;
+  bool fcur_stale = false;  // fcur predates surrogate-driven moves
   int slot = 0, potential = 0;
   double qbin = 0.0, conv = 1.0;
   Buf mass, mom, fcur, fnew, pos_save, kin0, ecur, ehist, acchist, counter, chains, gobs;
   int64_t hist_cap = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int g_leap = -1;
-  double g_eps = 0.0, g_beta = 0.0;
-  uint64_t g_seed = 0, g_stamp = 0;
-  int64_t g_cap = 0;
+  GraphSlot exact_g, sur_g;
+};
+
+// Learned integrator state (network, neighbour lists, output-layer fit).
+struct Sur {
+  bool set = false, w3_set = false;
+  float rbf[4] = {0.f, 0.f, 0.f, 0.f};  // rc, mu0, dmu, eta
+  float skin = 0.5f;
+  int max_nbr = 0;
+  Buf net, w3, onehot, nbr, cnt, d, cbuf, eatom, fsur, phi, gpart, gout, viol, viol_count,
+      atom_chain, tiles64, tiles256, chunks, granges;
+  int n_t64 = 0, n_t256 = 0, n_chunks = 0;
+  uint64_t stamp = ~0ull;
+  std::vector<double> acc;  // per chain: accumulated Phi Phi^T, Phi y, y.y (FP64)
+  std::vector<double> r2;   // fit quality of the last training sample per chain
 };
 
 }  // namespace
@@ -143,6 +170,7 @@ struct pyiid_ctx {
   Buf gpart, fpart, forcepart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, gobs;
 
   Hmc hmc;
+  Sur sur;
   bool capturing = false;
 
   // timing: event pairs recorded around launches, resolved when queried
@@ -501,12 +529,17 @@ int check_pdf_setup(pyiid_ctx* c, int slot, int potential) {
   return PYIID_OK;
 }
 
+void destroy_slot(GraphSlot& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+  g.leap = -1;
+}
+
 void destroy_graph(Hmc& h) {
-  if (h.exec) (void)hipGraphExecDestroy(h.exec);
-  if (h.graph) (void)hipGraphDestroy(h.graph);
-  h.exec = nullptr;
-  h.graph = nullptr;
-  h.g_leap = -1;
+  destroy_slot(h.exact_g);
+  destroy_slot(h.sur_g);
 }
 
 int set_structure(pyiid_ctx* c, const float* pos, int64_t n, const std::vector<int64_t>& off,
@@ -562,9 +595,364 @@ int hmc_proposal(pyiid_ctx* c, int n_leapfrog, double eps, double beta, uint64_t
                         h.fnew.as<double>(), h.ecur.as<double>(), c->out5.as<double>(), h.conv,
                         h.chains.as<HmcChain>(), B, beta, seed, h.counter.as<int64_t>(),
                         h.kin0.as<double>(), h.ehist.as<double>(), h.acchist.as<int32_t>(),
-                        h.hist_cap));
+                        h.hist_cap, nullptr, nullptr));
   return PYIID_OK;
 }
+
+// ---- learned integrator ----
+
+constexpr int kGramChunk = 2048;  // flat force columns per Gram partial
+
+uint16_t host_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+// Tiles, chunk lists and per-atom buffers of the surrogate for the current
+// batch layout.  The output-layer fit restarts when the layout changes.
+int ensure_sur_layout(pyiid_ctx* c) {
+  Sur& s = c->sur;
+  if (s.stamp == c->struct_stamp) return PYIID_OK;
+  if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  const int B = c->n_chain;
+  const int64_t n = c->n;
+  std::vector<Item> t64, t256, chunks;
+  std::vector<Range> gr(B);
+  std::vector<int32_t> ach(static_cast<size_t>(std::max<int64_t>(1, n)), 0);
+  int slot = 0;
+  for (int b = 0; b < B; ++b) {
+    const int o = static_cast<int>(c->off[b]), e = static_cast<int>(c->off[b + 1]);
+    for (int ib = o; ib < e; ib += 64) t64.push_back(Item{ib, std::min(e, ib + 64), o, e, 0, b, 0, 0});
+    for (int ib = o; ib < e; ib += 256)
+      t256.push_back(Item{ib, std::min(e, ib + 256), o, e, 0, b, 0, 0});
+    for (int i = o; i < e; ++i) ach[i] = b;
+    gr[b].begin = slot;
+    for (int64_t c0 = 3LL * o; c0 < 3LL * e; c0 += kGramChunk)
+      chunks.push_back(Item{0, 0, static_cast<int>(c0),
+                            static_cast<int>(std::min<int64_t>(3LL * e, c0 + kGramChunk)), slot++,
+                            b, 0, 0});
+    gr[b].end = slot;
+    gr[b].i_base = b;
+    gr[b].i_end = b;
+  }
+  const int64_t n1 = std::max<int64_t>(1, n);
+  CHK(s.tiles64.reserve(sizeof(Item) * std::max<size_t>(1, t64.size())));
+  CHK(s.tiles256.reserve(sizeof(Item) * std::max<size_t>(1, t256.size())));
+  CHK(s.chunks.reserve(sizeof(Item) * std::max<size_t>(1, chunks.size())));
+  CHK(s.granges.reserve(sizeof(Range) * B));
+  CHK(s.atom_chain.reserve(sizeof(int32_t) * n1));
+  CHK(s.nbr.reserve(sizeof(int32_t) * static_cast<size_t>(s.max_nbr) * n1));
+  CHK(s.cnt.reserve(sizeof(int32_t) * n1));
+  CHK(s.d.reserve(sizeof(float) * kSurRbf * n1));
+  CHK(s.cbuf.reserve(sizeof(float) * kSurRbf * n1));
+  CHK(s.eatom.reserve(sizeof(float) * n1));
+  CHK(s.fsur.reserve(sizeof(double) * 3 * n1));
+  CHK(s.gpart.reserve(sizeof(float) * kSurGram * std::max<size_t>(1, chunks.size())));
+  CHK(s.gout.reserve(sizeof(double) * kSurGram * B));
+  CHK(s.viol.reserve(sizeof(int32_t) * B));
+  CHK(s.viol_count.reserve(sizeof(int32_t) * B));
+  CHK(s.w3.reserve(sizeof(uint16_t) * kSurHidden * B));
+  if (!t64.empty())
+    HIPCHK(hipMemcpy(s.tiles64.p, t64.data(), sizeof(Item) * t64.size(), hipMemcpyHostToDevice));
+  if (!t256.empty())
+    HIPCHK(hipMemcpy(s.tiles256.p, t256.data(), sizeof(Item) * t256.size(), hipMemcpyHostToDevice));
+  if (!chunks.empty())
+    HIPCHK(hipMemcpy(s.chunks.p, chunks.data(), sizeof(Item) * chunks.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s.granges.p, gr.data(), sizeof(Range) * B, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s.atom_chain.p, ach.data(), sizeof(int32_t) * n1, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(s.viol.p, 0, sizeof(int32_t) * B));
+  HIPCHK(hipMemset(s.viol_count.p, 0, sizeof(int32_t) * B));
+  s.n_t64 = static_cast<int>(t64.size());
+  s.n_t256 = static_cast<int>(t256.size());
+  s.n_chunks = static_cast<int>(chunks.size());
+  s.acc.assign(static_cast<size_t>(kSurGram) * B, 0.0);
+  s.r2.assign(B, 0.0);
+  s.w3_set = false;
+  s.stamp = c->struct_stamp;
+  return PYIID_OK;
+}
+
+int sur_list(pyiid_ctx* c) {
+  Sur& s = c->sur;
+  HIPCHK(launch_nbr_build(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n),
+                          s.tiles256.as<Item>(), s.n_t256, s.rbf[0] + s.skin, s.max_nbr,
+                          s.nbr.as<int>(), s.cnt.as<int>(), s.viol.as<int>()));
+  return PYIID_OK;
+}
+
+// Surrogate forces (or one feature-force column) from the current list.
+int sur_eval(pyiid_ctx* c, const uint16_t* w3, int w3_stride, double* force, float* phi,
+             float* eatom, bool descriptors = true) {
+  Sur& s = c->sur;
+  const int n = static_cast<int>(c->n);
+  if (descriptors)
+    HIPCHK(launch_descriptors(c->stream, c->pos.as<AtomRec>(), n, s.nbr.as<int>(),
+                              s.cnt.as<int>(), s.rbf, s.d.as<float>()));
+  HIPCHK(launch_mlp(c->stream, s.d.as<float>(), n, s.tiles64.as<Item>(), s.n_t64, s.net.p, w3,
+                    w3_stride, s.cbuf.as<float>(), eatom));
+  HIPCHK(launch_sur_force(c->stream, c->pos.as<AtomRec>(), n, s.nbr.as<int>(), s.cnt.as<int>(),
+                          s.rbf, s.cbuf.as<float>(), force, phi));
+  return PYIID_OK;
+}
+
+// Raises if a neighbour list overflowed (clears the flags).
+int sur_check_lists(pyiid_ctx* c) {
+  Sur& s = c->sur;
+  std::vector<int32_t> v(c->n_chain);
+  HIPCHK(hipMemcpyAsync(v.data(), s.viol.p, sizeof(int32_t) * v.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < c->n_chain; ++b)
+    if (v[b]) {
+      HIPCHK(hipMemset(s.viol.p, 0, sizeof(int32_t) * c->n_chain));
+      return fail(PYIID_E_INVALID,
+                  "chain %d: an atom has more than %d neighbours within %g A (raise max_nbr)", b,
+                  s.max_nbr, static_cast<double>(s.rbf[0] + s.skin));
+    }
+  return PYIID_OK;
+}
+
+// Feature forces Phi[k][3i + c] = -d/dx_ic sum_a h2_k(D_a) for the 64
+// output units (the forces of the network with w3 = e_k).
+int sur_features(pyiid_ctx* c) {
+  Sur& s = c->sur;
+  const int64_t n3 = 3 * c->n;
+  CHK(s.phi.reserve(sizeof(float) * kSurHidden * static_cast<size_t>(std::max<int64_t>(1, n3))));
+  CHK(sur_list(c));
+  CHK(sur_check_lists(c));
+  HIPCHK(launch_descriptors(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n),
+                            s.nbr.as<int>(), s.cnt.as<int>(), s.rbf, s.d.as<float>()));
+  for (int k = 0; k < kSurHidden; ++k)
+    CHK(sur_eval(c, s.onehot.as<uint16_t>() + kSurHidden * k, 0, nullptr,
+                 s.phi.as<float>() + k * n3, nullptr, false));
+  return PYIID_OK;
+}
+
+// Solves (A + lam I) w = r in place by Cholesky (FP64); false if not SPD.
+bool cholesky_solve(std::vector<double>& a, std::vector<double>& r, int n) {
+  for (int j = 0; j < n; ++j) {
+    double d = a[j * n + j];
+    for (int k = 0; k < j; ++k) d -= a[j * n + k] * a[j * n + k];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    a[j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double v = a[i * n + j];
+      for (int k = 0; k < j; ++k) v -= a[i * n + k] * a[j * n + k];
+      a[i * n + j] = v / d;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double v = r[i];
+    for (int k = 0; k < i; ++k) v -= a[i * n + k] * r[k];
+    r[i] = v / a[i * n + i];
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double v = r[i];
+    for (int k = i + 1; k < n; ++k) v -= a[k * n + i] * r[k];
+    r[i] = v / a[i * n + i];
+  }
+  return true;
+}
+
+// One training step at the current state: exact energy and forces (they also
+// refresh the sampler's state), feature forces, the chains' normal equations
+// added to the running ones (decayed by `forget`), ridge solve for w3.
+int sur_train_step(pyiid_ctx* c, double forget, double ridge) {
+  Hmc& h = c->hmc;
+  Sur& s = c->sur;
+  const int B = c->n_chain;
+  CHK(ensure_sur_layout(c));
+  CHK(energy_forces_device(c, h.slot, h.qbin, h.potential, h.gobs.as<double>(), 2.0 * h.conv,
+                           h.fcur.as<double>()));
+  HIPCHK(launch_energy_from_out5(c->stream, c->out5.as<double>(), B, h.conv, h.ecur.as<double>()));
+  h.fcur_stale = false;
+  CHK(sur_features(c));
+  HIPCHK(launch_gram(c->stream, s.phi.as<float>(), 3 * c->n, h.fcur.as<double>(),
+                     s.chunks.as<Item>(), s.n_chunks, s.gpart.as<float>(), s.granges.as<Range>(),
+                     B, s.gout.as<double>()));
+  std::vector<double> g(static_cast<size_t>(kSurGram) * B);
+  HIPCHK(hipMemcpyAsync(g.data(), s.gout.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  constexpr int H = kSurHidden;
+  std::vector<uint16_t> w3h(static_cast<size_t>(H) * B);
+  std::vector<double> a(H * H), w(H);
+  for (int b = 0; b < B; ++b) {
+    double* acc = &s.acc[static_cast<size_t>(b) * kSurGram];
+    const double* gn = &g[static_cast<size_t>(b) * kSurGram];
+    for (int e = 0; e < kSurGram; ++e) acc[e] = forget * acc[e] + gn[e];
+    double tr = 0.0;
+    for (int k = 0; k < H; ++k) tr += acc[k * H + k];
+    std::fill(w.begin(), w.end(), 0.0);
+    if (tr > 0.0) {
+      double lam = std::max(ridge, 1e-12) * tr / H;
+      for (int attempt = 0; attempt < 8; ++attempt, lam *= 10.0) {
+        for (int e = 0; e < H * H; ++e) a[e] = acc[e];
+        for (int k = 0; k < H; ++k) a[k * H + k] += lam;
+        for (int k = 0; k < H; ++k) w[k] = acc[H * H + k];
+        if (cholesky_solve(a, w, H)) break;
+        std::fill(w.begin(), w.end(), 0.0);
+      }
+    }
+    // fit quality on this sample: 1 - |y - Phi w|^2 / |y|^2
+    const double yy = gn[H * H + H];
+    double res = yy;
+    for (int k = 0; k < H; ++k) {
+      res -= 2.0 * w[k] * gn[H * H + k];
+      double t = 0.0;
+      for (int l = 0; l < H; ++l) t += gn[k * H + l] * w[l];
+      res += w[k] * t;
+    }
+    s.r2[b] = yy > 0.0 ? 1.0 - res / yy : 0.0;
+    for (int k = 0; k < H; ++k) w3h[static_cast<size_t>(b) * H + k] = host_bf16(static_cast<float>(w[k]));
+  }
+  HIPCHK(hipMemcpy(s.w3.p, w3h.data(), sizeof(uint16_t) * w3h.size(), hipMemcpyHostToDevice));
+  s.w3_set = true;
+  return PYIID_OK;
+}
+
+// One proposal driven by surrogate forces; the exact energy of the end point
+// decides acceptance.  Fixed launch sequence (capturable).
+int sur_proposal(pyiid_ctx* c, int n_leapfrog, double eps, double beta, uint64_t seed) {
+  Hmc& h = c->hmc;
+  Sur& s = c->sur;
+  const int B = c->n_chain;
+  const uint16_t* w3 = s.w3.as<uint16_t>();
+  CHK(sur_list(c));
+  CHK(sur_eval(c, w3, kSurHidden, s.fsur.as<double>(), nullptr, nullptr));
+  HIPCHK(launch_hmc_begin(c->stream, c->pos.as<AtomRec>(), h.pos_save.as<double>(),
+                          h.mom.as<double>(), s.fsur.as<double>(), h.mass.as<double>(),
+                          h.chains.as<HmcChain>(), B, beta, eps, seed, h.counter.as<int64_t>(),
+                          h.kin0.as<double>()));
+  for (int l = 1; l <= n_leapfrog; ++l) {
+    HIPCHK(launch_hmc_drift(c->stream, c->pos.as<AtomRec>(), h.mom.as<double>(),
+                            h.mass.as<double>(), c->n, eps));
+    HIPCHK(launch_disp_check(c->stream, c->pos.as<AtomRec>(), h.pos_save.as<double>(),
+                             s.atom_chain.as<int>(), c->n, 0.5 * s.skin, s.viol.as<int>()));
+    CHK(sur_eval(c, w3, kSurHidden, s.fsur.as<double>(), nullptr, nullptr));
+    HIPCHK(launch_hmc_kick(c->stream, h.mom.as<double>(), s.fsur.as<double>(), c->n,
+                           l < n_leapfrog ? eps : 0.5 * eps));
+  }
+  CHK(energy_forces_device(c, h.slot, h.qbin, h.potential, h.gobs.as<double>(), 0.0, nullptr));
+  HIPCHK(launch_hmc_end(c->stream, c->pos.as<AtomRec>(), h.pos_save.as<double>(),
+                        h.mom.as<double>(), h.mass.as<double>(), nullptr, nullptr,
+                        h.ecur.as<double>(), c->out5.as<double>(), h.conv,
+                        h.chains.as<HmcChain>(), B, beta, seed, h.counter.as<int64_t>(),
+                        h.kin0.as<double>(), h.ehist.as<double>(), h.acchist.as<int32_t>(),
+                        h.hist_cap, s.viol.as<int>(), s.viol_count.as<int>()));
+  h.fcur_stale = true;
+  return PYIID_OK;
+}
+
+// Runs one proposal of either kind, from a captured graph when asked: the
+// first call with new parameters (or after any workspace moved) runs the
+// proposal directly, then captures it for the following calls.
+int run_proposal(pyiid_ctx* c, bool surrogate, bool use_graph, int n_leapfrog, double eps,
+                 double beta, uint64_t seed) {
+  Hmc& h = c->hmc;
+  auto body = [&]() {
+    return surrogate ? sur_proposal(c, n_leapfrog, eps, beta, seed)
+                     : hmc_proposal(c, n_leapfrog, eps, beta, seed);
+  };
+  if (!use_graph) return body();
+  GraphSlot& g = surrogate ? h.sur_g : h.exact_g;
+  const bool same = g.exec && g.leap == n_leapfrog && g.eps == eps && g.beta == beta &&
+                    g.seed == seed && g.stamp == c->struct_stamp && g.cap == h.hist_cap &&
+                    g.epoch == g_alloc_epoch.load();
+  if (same) {
+    HIPCHK(hipGraphLaunch(g.exec, c->stream));
+    if (surrogate) h.fcur_stale = true;
+    return PYIID_OK;
+  }
+  destroy_slot(g);
+  CHK(body());  // this proposal; it also sizes every workspace
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  const int rc = body();  // recorded, not executed
+  c->capturing = false;
+  hipGraph_t gr = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &gr);
+  if (rc != PYIID_OK) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+  }
+  HIPCHK(e);
+  g.graph = gr;
+  HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  g.leap = n_leapfrog;
+  g.eps = eps;
+  g.beta = beta;
+  g.seed = seed;
+  g.stamp = c->struct_stamp;
+  g.cap = h.hist_cap;
+  g.epoch = g_alloc_epoch.load();
+  return PYIID_OK;
+}
+
+// Exact energy and forces of the current state when surrogate moves made
+// the sampler's cached ones stale.
+int refresh_exact(pyiid_ctx* c) {
+  Hmc& h = c->hmc;
+  if (!h.fcur_stale) return PYIID_OK;
+  CHK(energy_forces_device(c, h.slot, h.qbin, h.potential, h.gobs.as<double>(), 2.0 * h.conv,
+                           h.fcur.as<double>()));
+  HIPCHK(launch_energy_from_out5(c->stream, c->out5.as<double>(), c->n_chain, h.conv,
+                                 h.ecur.as<double>()));
+  h.fcur_stale = false;
+  return PYIID_OK;
+}
+
+int prepare_history(pyiid_ctx* c, int n_proposals) {
+  Hmc& h = c->hmc;
+  const int B = c->n_chain;
+  if (n_proposals > h.hist_cap) {
+    destroy_graph(h);  // graphs record the history buffers' addresses
+    CHK(h.ehist.reserve(sizeof(double) * (size_t)n_proposals * B));
+    CHK(h.acchist.reserve(sizeof(int32_t) * (size_t)n_proposals * B));
+    h.hist_cap = n_proposals;
+  }
+  const int64_t zero = 0;
+  HIPCHK(hipMemcpyAsync(h.counter.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PYIID_OK;
+}
+
+// Per-proposal completion times (ms since the run's start) from events
+// recorded on the stream after each proposal.
+struct RunClock {
+  pyiid_ctx* c;
+  hipEvent_t start = nullptr;
+  std::vector<hipEvent_t> ev;
+  explicit RunClock(pyiid_ctx* c_, bool on) : c(c_) {
+    if (on && (start = take_event(c))) (void)hipEventRecord(start, c->stream);
+  }
+  int mark() {
+    if (!start) return PYIID_OK;
+    hipEvent_t e = take_event(c);
+    if (!e) return fail(PYIID_E_HIP, "hipEventCreate failed");
+    HIPCHK(hipEventRecord(e, c->stream));
+    ev.push_back(e);
+    return PYIID_OK;
+  }
+  int collect(double* out) {
+    if (!start) return PYIID_OK;
+    for (size_t k = 0; k < ev.size(); ++k) {
+      HIPCHK(hipEventSynchronize(ev[k]));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, start, ev[k]));
+      if (out) out[k] = ms;
+    }
+    return PYIID_OK;
+  }
+  ~RunClock() {
+    if (start) c->ev_pool.push_back(start);
+    for (hipEvent_t e : ev) c->ev_pool.push_back(e);
+  }
+};
 
 }  // namespace
 
@@ -615,7 +1003,11 @@ int pyiid_destroy(pyiid_ctx* c) {
                  &c->fq_items.ranges, &c->fused_items.items, &c->fused_items.ranges, &c->hmc.mass,
                  &c->hmc.mom, &c->hmc.fcur, &c->hmc.fnew, &c->hmc.pos_save, &c->hmc.kin0,
                  &c->hmc.ecur, &c->hmc.ehist, &c->hmc.acchist, &c->hmc.counter, &c->hmc.chains,
-                 &c->hmc.gobs})
+                 &c->hmc.gobs, &c->sur.net, &c->sur.w3, &c->sur.onehot, &c->sur.nbr, &c->sur.cnt,
+                 &c->sur.d, &c->sur.cbuf, &c->sur.eatom, &c->sur.fsur, &c->sur.phi,
+                 &c->sur.gpart, &c->sur.gout, &c->sur.viol, &c->sur.viol_count,
+                 &c->sur.atom_chain, &c->sur.tiles64, &c->sur.tiles256, &c->sur.chunks,
+                 &c->sur.granges})
     b->release();
   for (Table& t : c->tab) {
     t.tab.release();
@@ -1100,11 +1492,14 @@ int pyiid_hmc_setup(pyiid_ctx* c, int slot, double qbin, const double* gobs, int
   HIPCHK(launch_energy_from_out5(c->stream, c->out5.as<double>(), B, conv, h.ecur.as<double>()));
   HIPCHK(hipStreamSynchronize(c->stream));  // `ch` is a host temporary
   h.set = true;
+  h.fcur_stale = false;
+  c->sur.stamp = ~0ull;  // a new target: the surrogate fit starts again
   return PYIID_OK;
 }
 
 int pyiid_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step, double beta,
-                  uint64_t seed, int use_graph, double* energies, int32_t* accepted, int flags) {
+                  uint64_t seed, int use_graph, double* energies, int32_t* accepted,
+                  double* times_ms, int flags) {
   CHK(check_ctx(c));
   Hmc& h = c->hmc;
   if (!h.set) return fail(PYIID_E_STATE, "call pyiid_hmc_setup first");
@@ -1112,53 +1507,176 @@ int pyiid_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step, do
     return fail(PYIID_E_INVALID, "need n_proposals >= 0, n_leapfrog >= 1, beta > 0");
   const int B = c->n_chain;
   if (n_proposals == 0) return PYIID_OK;
-  if (n_proposals > h.hist_cap) {
-    destroy_graph(h);  // the graph records the history buffers' addresses
-    CHK(h.ehist.reserve(sizeof(double) * (size_t)n_proposals * B));
-    CHK(h.acchist.reserve(sizeof(int32_t) * (size_t)n_proposals * B));
-    h.hist_cap = n_proposals;
+  CHK(prepare_history(c, n_proposals));
+  CHK(refresh_exact(c));
+  RunClock clock(c, times_ms != nullptr);
+  for (int p = 0; p < n_proposals; ++p) {
+    CHK(run_proposal(c, false, use_graph != 0, n_leapfrog, step, beta, seed));
+    CHK(clock.mark());
   }
-  const int64_t zero = 0;
-  HIPCHK(hipMemcpyAsync(h.counter.p, &zero, sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  int done = 0;
-  if (use_graph) {
-    const bool same = h.exec && h.g_leap == n_leapfrog && h.g_eps == step && h.g_beta == beta &&
-                      h.g_seed == seed && h.g_stamp == c->struct_stamp && h.g_cap == h.hist_cap;
-    if (!same) {
-      destroy_graph(h);
-      // one uncaptured proposal warms every buffer and cache, then capture
-      CHK(hmc_proposal(c, n_leapfrog, step, beta, seed));
-      done = 1;
-      HIPCHK(hipStreamSynchronize(c->stream));
-      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      c->capturing = true;
-      const int rc = hmc_proposal(c, n_leapfrog, step, beta, seed);
-      c->capturing = false;
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (rc != PYIID_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      HIPCHK(e);
-      h.graph = g;
-      HIPCHK(hipGraphInstantiate(&h.exec, h.graph, nullptr, nullptr, 0));
-      h.g_leap = n_leapfrog;
-      h.g_eps = step;
-      h.g_beta = beta;
-      h.g_seed = seed;
-      h.g_stamp = c->struct_stamp;
-      h.g_cap = h.hist_cap;
-    }
-    for (int p = done; p < n_proposals; ++p) HIPCHK(hipGraphLaunch(h.exec, c->stream));
-  } else {
-    for (int p = 0; p < n_proposals; ++p) CHK(hmc_proposal(c, n_leapfrog, step, beta, seed));
-  }
+  CHK(clock.collect(times_ms));
   if (energies)
     CHK(download(c, energies, h.ehist.p, sizeof(double) * (size_t)n_proposals * B, flags));
   if (accepted)
     CHK(download(c, accepted, h.acchist.p, sizeof(int32_t) * (size_t)n_proposals * B, flags));
+  return finish(c, flags);
+}
+
+int pyiid_sur_setup(pyiid_ctx* c, const float* params, const float* w1, const float* b1,
+                    const float* w2, const float* b2, const float* d_mean,
+                    const float* d_inv_scale, int32_t max_nbr) {
+  CHK(check_ctx(c));
+  if (!params || !w1 || !b1 || !w2 || !b2 || !d_mean || !d_inv_scale)
+    return fail(PYIID_E_INVALID, "null network parameter");
+  const float rc = params[0], skin = params[1], mu0 = params[2], dmu = params[3],
+              eta = params[4];
+  if (!(rc > 0.f) || !(skin > 0.f) || !(dmu > 0.f) || !(eta > 0.f) || !std::isfinite(mu0))
+    return fail(PYIID_E_INVALID, "need rc, skin, dmu, eta > 0");
+  if (max_nbr < 1 || max_nbr > 4096) return fail(PYIID_E_INVALID, "max_nbr %d not in [1, 4096]", max_nbr);
+  Sur& s = c->sur;
+  destroy_graph(c->hmc);
+  std::vector<unsigned char> blob(kSurNetBytes);
+  unsigned char* p = blob.data();
+  for (int e = 0; e < 16 * 64; ++e, p += 2) {
+    const uint16_t v = host_bf16(w1[e]);
+    std::memcpy(p, &v, 2);
+  }
+  for (int e = 0; e < 64 * 64; ++e, p += 2) {
+    const uint16_t v = host_bf16(w2[e]);
+    std::memcpy(p, &v, 2);
+  }
+  std::memcpy(p, b1, 64 * 4), p += 256;
+  std::memcpy(p, b2, 64 * 4), p += 256;
+  std::memcpy(p, d_mean, 16 * 4), p += 64;
+  std::memcpy(p, d_inv_scale, 16 * 4), p += 64;
+  CHK(s.net.reserve(kSurNetBytes));
+  HIPCHK(hipMemcpy(s.net.p, blob.data(), kSurNetBytes, hipMemcpyHostToDevice));
+  std::vector<uint16_t> onehot(kSurHidden * kSurHidden, 0);
+  for (int k = 0; k < kSurHidden; ++k) onehot[k * kSurHidden + k] = host_bf16(1.0f);
+  CHK(s.onehot.reserve(sizeof(uint16_t) * onehot.size()));
+  HIPCHK(hipMemcpy(s.onehot.p, onehot.data(), sizeof(uint16_t) * onehot.size(),
+                   hipMemcpyHostToDevice));
+  s.rbf[0] = rc;
+  s.rbf[1] = mu0;
+  s.rbf[2] = dmu;
+  s.rbf[3] = eta;
+  s.skin = skin;
+  s.max_nbr = max_nbr;
+  s.stamp = ~0ull;  // buffers are sized with max_nbr
+  s.set = true;
+  return PYIID_OK;
+}
+
+static int sur_ready(pyiid_ctx* c) {
+  CHK(check_ctx(c));
+  if (!c->sur.set) return fail(PYIID_E_STATE, "call pyiid_sur_setup first");
+  if (!c->pos_set) return fail(PYIID_E_STATE, "positions not set");
+  CHK(ensure_packed(c));
+  return ensure_sur_layout(c);
+}
+
+int pyiid_sur_set_output(pyiid_ctx* c, const float* w3, int flags) {
+  CHK(sur_ready(c));
+  if (!w3) return fail(PYIID_E_INVALID, "null w3");
+  std::vector<float> w(static_cast<size_t>(kSurHidden) * c->n_chain);
+  if (flags & PYIID_DEVICE) {
+    HIPCHK(hipMemcpy(w.data(), w3, sizeof(float) * w.size(), hipMemcpyDeviceToHost));
+  } else {
+    std::memcpy(w.data(), w3, sizeof(float) * w.size());
+  }
+  std::vector<uint16_t> h(w.size());
+  for (size_t e = 0; e < w.size(); ++e) h[e] = host_bf16(w[e]);
+  HIPCHK(hipMemcpy(c->sur.w3.p, h.data(), sizeof(uint16_t) * h.size(), hipMemcpyHostToDevice));
+  c->sur.w3_set = true;
+  return PYIID_OK;
+}
+
+int pyiid_sur_descriptors(pyiid_ctx* c, float* d, int flags) {
+  CHK(sur_ready(c));
+  if (!d) return fail(PYIID_E_INVALID, "null output");
+  Sur& s = c->sur;
+  CHK(sur_list(c));
+  CHK(sur_check_lists(c));
+  HIPCHK(launch_descriptors(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n),
+                            s.nbr.as<int>(), s.cnt.as<int>(), s.rbf, s.d.as<float>()));
+  if (flags & PYIID_DEVICE) {
+    HIPCHK(hipMemcpyAsync(d, s.d.p, sizeof(float) * kSurRbf * c->n, hipMemcpyDeviceToDevice,
+                          c->stream));
+  } else {
+    CHK(download(c, d, s.d.p, sizeof(float) * kSurRbf * c->n, flags));
+  }
+  return finish(c, flags);
+}
+
+int pyiid_sur_forces(pyiid_ctx* c, double* forces, float* atom_energy, int flags) {
+  CHK(sur_ready(c));
+  Sur& s = c->sur;
+  if (!s.w3_set) return fail(PYIID_E_STATE, "surrogate output layer not set (train or set it)");
+  CHK(sur_list(c));
+  CHK(sur_check_lists(c));
+  CHK(sur_eval(c, s.w3.as<uint16_t>(), kSurHidden, s.fsur.as<double>(), nullptr,
+               s.eatom.as<float>()));
+  if (forces) CHK(download(c, forces, s.fsur.p, sizeof(double) * 3 * c->n, flags));
+  if (atom_energy) CHK(download(c, atom_energy, s.eatom.p, sizeof(float) * c->n, flags));
+  return finish(c, flags);
+}
+
+int pyiid_sur_features(pyiid_ctx* c, float* phi, int flags) {
+  CHK(sur_ready(c));
+  if (!phi) return fail(PYIID_E_INVALID, "null output");
+  CHK(sur_features(c));
+  CHK(download(c, phi, c->sur.phi.p, sizeof(float) * kSurHidden * 3 * c->n, flags));
+  return finish(c, flags);
+}
+
+int pyiid_sur_train(pyiid_ctx* c, double forget, double ridge, double* r2, int flags) {
+  CHK(sur_ready(c));
+  if (!c->hmc.set) return fail(PYIID_E_STATE, "call pyiid_hmc_setup first (targets)");
+  if (!(forget >= 0.0 && forget <= 1.0) || !(ridge >= 0.0))
+    return fail(PYIID_E_INVALID, "need 0 <= forget <= 1 and ridge >= 0");
+  CHK(sur_train_step(c, forget, ridge));
+  if (r2) std::memcpy(r2, c->sur.r2.data(), sizeof(double) * c->n_chain);
+  (void)flags;
+  return PYIID_OK;
+}
+
+int pyiid_sur_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step, double beta,
+                      uint64_t seed, int use_graph, int train_every, double forget, double ridge,
+                      double min_r2, double* energies, int32_t* accepted, int32_t* exact_mode,
+                      double* times_ms, int32_t* violations, int flags) {
+  CHK(sur_ready(c));
+  Hmc& h = c->hmc;
+  Sur& s = c->sur;
+  if (!h.set) return fail(PYIID_E_STATE, "call pyiid_hmc_setup first");
+  if (n_proposals < 0 || n_leapfrog < 1 || !(beta > 0.0) || train_every < 0)
+    return fail(PYIID_E_INVALID,
+                "need n_proposals >= 0, n_leapfrog >= 1, beta > 0, train_every >= 0");
+  if (!(forget >= 0.0 && forget <= 1.0) || !(ridge >= 0.0))
+    return fail(PYIID_E_INVALID, "need 0 <= forget <= 1 and ridge >= 0");
+  if (min_chain_atoms(c) < 2) return fail(PYIID_E_INVALID, "every chain needs at least two atoms");
+  const int B = c->n_chain;
+  if (n_proposals == 0) return PYIID_OK;
+  CHK(prepare_history(c, n_proposals));
+  RunClock clock(c, times_ms != nullptr);
+  bool exact_block = false;
+  for (int p = 0; p < n_proposals; ++p) {
+    if (!s.w3_set || (train_every > 0 && p % train_every == 0)) {
+      CHK(sur_train_step(c, forget, ridge));
+      double mean = 0.0;
+      for (int b = 0; b < B; ++b) mean += s.r2[b];
+      exact_block = mean / B < min_r2;  // poor fit: exact forces until the next fit
+    }
+    if (exact_block) CHK(refresh_exact(c));
+    CHK(run_proposal(c, !exact_block, use_graph != 0, n_leapfrog, step, beta, seed));
+    if (exact_mode) exact_mode[p] = exact_block ? 1 : 0;
+    CHK(clock.mark());
+  }
+  CHK(clock.collect(times_ms));
+  if (energies)
+    CHK(download(c, energies, h.ehist.p, sizeof(double) * (size_t)n_proposals * B, flags));
+  if (accepted)
+    CHK(download(c, accepted, h.acchist.p, sizeof(int32_t) * (size_t)n_proposals * B, flags));
+  if (violations) CHK(download(c, violations, s.viol_count.p, sizeof(int32_t) * B, flags));
   return finish(c, flags);
 }
 

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void hmc_end_kernel(
     double* __restrict__ e_cur, const double* __restrict__ out5_new, double conv,
     const HmcChain* __restrict__ chains, int n_chain, double beta, uint64_t seed,
     const int64_t* __restrict__ counter, const double* __restrict__ kin0,
-    double* __restrict__ e_hist, int32_t* __restrict__ acc_hist, int64_t hist_cap) {
+    double* __restrict__ e_hist, int32_t* __restrict__ acc_hist, int64_t hist_cap,
+    int* __restrict__ viol, int* __restrict__ viol_count) {
   __shared__ double sh[8];
   __shared__ int accept_sh;
   const int b = blockIdx.x;
@@ -115,7 +116,12 @@ __global__ __launch_bounds__(256) void hmc_end_kernel(
     const double h0 = e_cur[b] + kin0[b];
     const double h1 = e_new + k;
     const double u = uniform_key(seed, cnt, 2, static_cast<uint64_t>(b));
-    const int acc = (u < exp(beta * (h0 - h1))) ? 1 : 0;
+    int acc = (u < exp(beta * (h0 - h1))) ? 1 : 0;
+    if (viol && viol[b]) {  // the proposal left its neighbour list's range
+      acc = 0;
+      viol[b] = 0;
+      viol_count[b] += 1;
+    }
     accept_sh = acc;
     if (acc) e_cur[b] = e_new;
     if (cnt < hist_cap) {
@@ -127,7 +133,8 @@ __global__ __launch_bounds__(256) void hmc_end_kernel(
   const int acc = accept_sh;
   for (int i = ch.atom_begin + threadIdx.x; i < ch.atom_end; i += blockDim.x) {
     if (acc) {
-      for (int c = 0; c < 3; ++c) force_cur[3 * (size_t)i + c] = force_new[3 * (size_t)i + c];
+      if (force_cur)
+        for (int c = 0; c < 3; ++c) force_cur[3 * (size_t)i + c] = force_new[3 * (size_t)i + c];
     } else {
       pos[i].x = pos_save[3 * (size_t)i + 0];
       pos[i].y = pos_save[3 * (size_t)i + 1];
@@ -174,10 +181,10 @@ hipError_t launch_hmc_end(hipStream_t st, AtomRec* pos, const double* pos_save, 
                           double* e_cur, const double* out5_new, double conv,
                           const HmcChain* chains, int n_chain, double beta, uint64_t seed,
                           int64_t* counter, const double* kin0, double* e_hist,
-                          int32_t* acc_hist, int64_t hist_cap) {
+                          int32_t* acc_hist, int64_t hist_cap, int* viol, int* viol_count) {
   hipLaunchKernelGGL(hmc_end_kernel, dim3(n_chain), dim3(256), 0, st, pos, pos_save, mom, mass,
                      force_cur, force_new, e_cur, out5_new, conv, chains, n_chain, beta, seed,
-                     counter, kin0, e_hist, acc_hist, hist_cap);
+                     counter, kin0, e_hist, acc_hist, hist_cap, viol, viol_count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(bump_kernel, dim3(1), dim3(1), 0, st, counter);

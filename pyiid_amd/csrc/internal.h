@@ -102,14 +102,43 @@ hipError_t launch_hmc_drift(hipStream_t st, AtomRec* pos, const double* mom, con
 hipError_t launch_hmc_kick(hipStream_t st, double* mom, const double* force, int64_t n, double h);
 // End of a proposal: K1, Metropolis with exp(beta (H0 - H1)), restore or
 // accept (positions, force, energy), record energy/acceptance, bump counter.
+// force_cur null: no force copy.  viol (nullable): chains flagged there are
+// rejected, counted in viol_count and their flag cleared.
 hipError_t launch_hmc_end(hipStream_t st, AtomRec* pos, const double* pos_save, const double* mom,
                           const double* mass, double* force_cur, const double* force_new,
                           double* e_cur, const double* out5_new, double conv,
                           const HmcChain* chains, int n_chain, double beta, uint64_t seed,
                           int64_t* counter, const double* kin0, double* e_hist,
-                          int32_t* acc_hist, int64_t hist_cap);
+                          int32_t* acc_hist, int64_t hist_cap, int* viol, int* viol_count);
 // e_cur[b] = conv * out5[b].value
 hipError_t launch_energy_from_out5(hipStream_t st, const double* out5, int n_chain, double conv,
                                    double* e);
+
+// ---- learned integrator (surrogate.hip) ----
+// Network parameter block: bf16 W1[16][64], bf16 W2[64][64], f32 b1[64],
+// b2[64], descriptor mean[16], inv_scale[16].
+constexpr int kSurNetBytes = 16 * 64 * 2 + 64 * 64 * 2 + 64 * 4 * 2 + 16 * 4 * 2;
+constexpr int kSurHidden = 64;
+constexpr int kSurRbf = 16;
+constexpr int kSurGram = 64 * 64 + 64 + 1;  // Phi Phi^T, Phi y, y.y per chain
+// tiles: Item{i_base, i_end, jbeg, jend = the chain's atoms, chain}.
+hipError_t launch_nbr_build(hipStream_t st, const AtomRec* pos, int n, const Item* tiles,
+                            int n_tiles, float rl, int max_nbr, int* nbr, int* cnt, int* viol);
+// rbf = {rc, mu0, dmu, eta}
+hipError_t launch_descriptors(hipStream_t st, const AtomRec* pos, int n, const int* nbr,
+                              const int* cnt, const float* rbf, float* d);
+hipError_t launch_mlp(hipStream_t st, const float* d, int n, const Item* tiles, int n_tiles,
+                      const void* net, const unsigned short* w3, int w3_stride, float* c,
+                      float* e_atom);
+hipError_t launch_sur_force(hipStream_t st, const AtomRec* pos, int n, const int* nbr,
+                            const int* cnt, const float* rbf, const float* c, double* force,
+                            float* phi);
+// chunks: Item{jbeg, jend = flat column range, split = partial slot, chain};
+// ranges[b] = {first slot, end slot}.  out[b] (FP64, kSurGram values).
+hipError_t launch_gram(hipStream_t st, const float* phi, int64_t n3, const double* y,
+                       const Item* chunks, int n_chunks, float* part, const Range* ranges,
+                       int n_chain, double* out);
+hipError_t launch_disp_check(hipStream_t st, const AtomRec* pos, const double* pos_save,
+                             const int* atom_chain, int64_t n, double lim, int* viol);
 
 }  // namespace pyiid

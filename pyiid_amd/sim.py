@@ -133,9 +133,12 @@ class DeviceHMC(object):
         targets = np.ascontiguousarray(targets, np.float64).reshape(self.B, len(r))
         self.ctx.hmc_setup(_lib.SLOT_PDF, scat.pdf_qbin, targets, self.pot, conv, m)
 
-    def run(self, n_proposals, n_leapfrog=10, step=2e-3, beta=1.0, seed=0, use_graph=True):
-        """Returns (energies, accepted), each (n_proposals, B)."""
-        return self.ctx.hmc_run(n_proposals, n_leapfrog, step, beta, seed, self.B, use_graph)
+    def run(self, n_proposals, n_leapfrog=10, step=2e-3, beta=1.0, seed=0, use_graph=True,
+            times=False):
+        """Returns (energies, accepted), each (n_proposals, B), plus the
+        proposals' completion times in ms when ``times``."""
+        return self.ctx.hmc_run(n_proposals, n_leapfrog, step, beta, seed, self.B, use_graph,
+                                times)
 
     def positions(self):
         """Current positions of every chain, as a list of (N_b, 3) float32 arrays."""
@@ -144,3 +147,38 @@ class DeviceHMC(object):
 
     def close(self):
         self.ctx.close()
+
+
+class SurrogateHMC(DeviceHMC):
+    """DeviceHMC whose proposals follow the learned integrator's forces.
+
+    Metropolis still uses the exact energy (conv * Rw from the Debye path), so
+    the stationary distribution is that of exact-force HMC.  The network's
+    output layer is refitted on exact forces at the chains' current states
+    every ``train_every`` proposals (pyiid_amd.surrogate)."""
+
+    def __init__(self, scat, structures, targets, potential="rw", conv=1.0, masses=None,
+                 device=0, net=None, net_seed=0, max_nbr=None):
+        from . import surrogate
+        DeviceHMC.__init__(self, scat, structures, targets, potential, conv, masses, device)
+        self.net = net if net is not None else surrogate.SurrogateNet(net_seed)
+        self.max_nbr = surrogate.MAX_NBR if max_nbr is None else int(max_nbr)
+        self.net.normalise(self.ctx, int(self.offsets[-1]), self.max_nbr)
+
+    def train(self, forget=0.0, ridge=1e-6):
+        """One fit at the current state; returns the per-chain r2."""
+        return self.ctx.sur_train(self.B, forget, ridge)
+
+    def run(self, n_proposals, n_leapfrog=10, step=2e-3, beta=1.0, seed=0, use_graph=True,
+            train_every=10, forget=0.5, ridge=1e-6, min_r2=-np.inf):
+        """Returns a dict of energies / accepted (n_proposals, B), exact_mode
+        (proposals run with exact forces), times_ms, violations."""
+        return self.ctx.sur_hmc_run(n_proposals, n_leapfrog, step, beta, seed, self.B, use_graph,
+                                    train_every, forget, ridge, min_r2)
+
+    def run_exact(self, n_proposals, n_leapfrog=10, step=2e-3, beta=1.0, seed=0, use_graph=True,
+                  times=False):
+        return DeviceHMC.run(self, n_proposals, n_leapfrog, step, beta, seed, use_graph, times)
+
+    def forces(self):
+        return self.ctx.sur_forces(int(self.offsets[-1]))
