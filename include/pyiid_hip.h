@@ -65,7 +65,7 @@ int pyiid_synchronize(pyiid_ctx *ctx);
 /* Kernel timing with HIP events on the launch stream: when enabled, every
  * launch of the Debye kernels is bracketed by events and its duration summed.
  * kind: 0 = fused F+grad kernel, 1 = F-only kernel, 2 = transform GEMM,
- *       3 = pair-geometry kernel. */
+ *       3 = pair-geometry kernel, 4 = contracted-force (unordered pair) kernel. */
 int pyiid_enable_timing(pyiid_ctx *ctx, int enable);
 int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launches);
 

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <utility>
 #include <vector>
 
@@ -165,9 +166,19 @@ struct pyiid_ctx {
 
   ItemSet fq_items;     // unordered pairs, ranges per chain
   ItemSet fused_items;  // ordered pairs with j splits, ranges per i tile
+  ItemSet pair_items;   // tile pairs (I, J <= I) of the contracted-force kernel
+  Buf pair_entries;     // (work, side) lists per tile
+  int pf_groups = 1, pf_span = 32;
+
+  // launch shapes (set by the autotuner per problem shape)
+  int grad_m = 32;             // Q bins per wave of the fused kernel
+  int64_t fused_waves = 32768; // target wave count that sets the fused j split
+  int64_t pf_waves = 8192;     // target wave count that sets the force Q groups
+  bool autotune = true;
+  std::map<std::vector<int64_t>, std::vector<int64_t>> tuned;
 
   // workspaces
-  Buf gpart, fpart, forcepart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, gobs;
+  Buf gpart, fpart, forcepart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, uq, gobs, pfpart;
 
   Hmc hmc;
   Sur sur;
@@ -177,8 +188,8 @@ struct pyiid_ctx {
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-  double t_ms[4] = {0, 0, 0, 0};
-  int64_t t_n[4] = {0, 0, 0, 0};
+  double t_ms[5] = {0, 0, 0, 0, 0};
+  int64_t t_n[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -348,8 +359,8 @@ int ensure_fused_items(pyiid_ctx* c, int nq) {
     n_tiles += (nb + 63) / 64;
     max_nb = std::max(max_nb, nb);
   }
-  const int64_t base = std::max<int64_t>(1, n_tiles * ((nq + kGradM - 1) / kGradM));
-  int64_t split = (32768 + base - 1) / base;
+  const int64_t base = std::max<int64_t>(1, n_tiles * ((nq + c->grad_m - 1) / c->grad_m));
+  int64_t split = (c->fused_waves + base - 1) / base;
   split = std::min<int64_t>(split, 32);
   split = std::min<int64_t>(split, std::max<int64_t>(1, max_nb / 64));
   split = std::max<int64_t>(1, split);
@@ -461,14 +472,17 @@ int compute_fq(pyiid_ctx* c, int slot, double qbin) {
 
 // Fused kernel over all chains: gradient partials (u == null) or contracted
 // force partials, plus F partials.
-int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u) {
+int tune_fused(pyiid_ctx* c, int slot, double qbin);
+
+int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, bool tune = true) {
   Table& t = c->tab[slot];
   const int nq = t.nq;
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
+  if (tune && !u) CHK(tune_fused(c, slot, qbin));
   CHK(ensure_fused_items(c, nq));
   const ItemSet& s = c->fused_items;
-  const int n_qchunk = ((nq + kWaves * kGradM - 1) / (kWaves * kGradM)) * kWaves;
+  const int n_qchunk = ((nq + kWaves * c->grad_m - 1) / (kWaves * c->grad_m)) * kWaves;
   CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, s.n_items) * nq));
   if (u) {
     CHK(c->forcepart.reserve(sizeof(float) * (size_t)std::max(1, s.n_items) * n_qchunk * 64 * 3));
@@ -477,11 +491,205 @@ int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u) {
   }
   const double dq = static_cast<double>(static_cast<float>(qbin));
   Timed tm(c, 0);
-  HIPCHK(launch_fused_grad(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n),
+  HIPCHK(launch_fused_grad(c->stream, c->grad_m, c->pos.as<AtomRec>(), static_cast<int>(c->n),
                            t.tab.as<float>(), t.n_elem, nq, dq, s.items.as<Item>(), s.n_items,
                            weighted_table(t), c->gpart.as<float>(), c->fpart.as<double>(), u,
                            c->forcepart.as<float>()));
   CHK(tm.done());
+  return PYIID_OK;
+}
+
+// Tile pairs (I, J <= I) of every chain for the contracted-force kernel, the
+// Q grouping (enough waves to fill the device, LDS within its limit) and the
+// (work, side) entry list of every tile for the deterministic reduction.
+int ensure_pair_items(pyiid_ctx* c, int nq, bool weighted, int n_elem) {
+  ItemSet& s = c->pair_items;
+  int64_t n_items = 0;
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int64_t t = (c->off[b + 1] - c->off[b] + 63) / 64;
+    n_items += t * (t + 1) / 2;
+  }
+  const int n_chunks = std::max(1, (nq + 31) / 32);
+  int64_t groups = (c->pf_waves + std::max<int64_t>(1, n_items) - 1) / std::max<int64_t>(1, n_items);
+  groups = std::max<int64_t>(1, std::min<int64_t>(groups, n_chunks));
+  const size_t table = weighted ? sizeof(float) * static_cast<size_t>(n_elem) * nq : 0;
+  int span = 0;
+  for (;; ++groups) {
+    const int per = static_cast<int>((n_chunks + groups - 1) / groups);
+    span = per * 32;
+    if (sizeof(float) * 2 * kWaves * static_cast<size_t>(span) + table <= kPairForceLds) break;
+    if (groups >= n_chunks) return fail(PYIID_E_INVALID, "contracted-force tables exceed LDS");
+  }
+  groups = (n_chunks + span / 32 - 1) / (span / 32);
+  if (n_items * groups > INT32_MAX / 2) return fail(PYIID_E_INVALID, "too many atom pairs");
+  const int key = static_cast<int>(groups * 100000 + span);
+  if (s.stamp == c->struct_stamp && s.key == key) return PYIID_OK;
+  if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  std::vector<Item> v;
+  std::vector<std::vector<int>> per_tile;
+  std::vector<Range> tiles;
+  int tile0 = 0;
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int o = static_cast<int>(c->off[b]), e = static_cast<int>(c->off[b + 1]);
+    const int nt = (e - o + 63) / 64;
+    for (int t = 0; t < nt; ++t) {
+      tiles.push_back(Range{0, 0, o + 64 * t, std::min(e, o + 64 * (t + 1))});
+      per_tile.emplace_back();
+    }
+    for (int ti = 0; ti < nt; ++ti)
+      for (int tj = 0; tj <= ti; ++tj) {
+        const int ib = o + 64 * ti, jb = o + 64 * tj;
+        const int k = static_cast<int>(v.size());
+        v.push_back(Item{ib, std::min(e, ib + 64), jb, std::min(e, jb + 64), 0, b, tile0 + ti,
+                         tile0 + tj});
+        for (int g = 0; g < groups; ++g) {
+          const int work = k * static_cast<int>(groups) + g;
+          per_tile[tile0 + ti].push_back(2 * work);
+          per_tile[tile0 + tj].push_back(2 * work + 1);
+        }
+      }
+    tile0 += nt;
+  }
+  std::vector<int> entries;
+  for (size_t t = 0; t < tiles.size(); ++t) {
+    tiles[t].begin = static_cast<int>(entries.size());
+    entries.insert(entries.end(), per_tile[t].begin(), per_tile[t].end());
+    tiles[t].end = static_cast<int>(entries.size());
+  }
+  CHK(upload_items(c, s, v, tiles));
+  CHK(c->pair_entries.reserve(sizeof(int) * std::max<size_t>(1, entries.size())));
+  if (!entries.empty())
+    HIPCHK(hipMemcpy(c->pair_entries.p, entries.data(), sizeof(int) * entries.size(),
+                     hipMemcpyHostToDevice));
+  c->pf_groups = static_cast<int>(groups);
+  c->pf_span = span;
+  s.key = key;
+  s.stamp = c->struct_stamp;
+  return PYIID_OK;
+}
+
+int launch_contract(pyiid_ctx* c, int slot, double qbin) {
+  Table& t = c->tab[slot];
+  const int nq = t.nq;
+  const double dq = static_cast<double>(static_cast<float>(qbin));
+  CHK(ensure_pair_items(c, nq, weighted_table(t), t.n_elem));
+  const ItemSet& s = c->pair_items;
+  CHK(c->pfpart.reserve(sizeof(float) * 384 * static_cast<size_t>(std::max(1, s.n_items)) *
+                        c->pf_groups));
+  HIPCHK(launch_pair_force(c->stream, c->pos.as<AtomRec>(), t.tab.as<float>(), t.n_elem, nq, dq,
+                           s.items.as<Item>(), s.n_items, c->pf_groups, c->pf_span,
+                           weighted_table(t), c->u.as<float>(), c->uq.as<float>(),
+                           c->pfpart.as<float>()));
+  return PYIID_OK;
+}
+
+// Times one launch of `fn` on the context stream with HIP events (ms).
+template <typename F>
+int time_launch(pyiid_ctx* c, F fn, double* ms) {
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return fail(PYIID_E_HIP, "hipEventCreate failed");
+  HIPCHK(hipEventRecord(a, c->stream));
+  const int rc = fn();
+  HIPCHK(hipEventRecord(b, c->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  c->ev_pool.push_back(a);
+  c->ev_pool.push_back(b);
+  *ms = t;
+  return rc;
+}
+
+// Autotuner: the first call for a problem shape runs every candidate launch
+// shape once to warm it and once timed, keeps the fastest, and remembers it
+// for the shape.  Never runs while a graph is captured (defaults apply).
+int tune_contract(pyiid_ctx* c, int slot, double qbin) {
+  Table& t = c->tab[slot];
+  const std::vector<int64_t> key{1, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
+                                 static_cast<int64_t>(c->struct_stamp)};
+  auto hit = c->tuned.find(key);
+  if (hit != c->tuned.end()) {
+    c->pf_waves = hit->second[0];
+    return PYIID_OK;
+  }
+  if (c->capturing || !c->autotune) return PYIID_OK;
+  double best = 1e300;
+  int64_t best_w = c->pf_waves;
+  for (int64_t w : {2048LL, 8192LL, 32768LL}) {
+    c->pf_waves = w;
+    double ms = 0.0;
+    CHK(launch_contract(c, slot, qbin));
+    CHK(time_launch(c, [&]() { return launch_contract(c, slot, qbin); }, &ms));
+    if (ms < best) {
+      best = ms;
+      best_w = w;
+    }
+  }
+  c->pf_waves = best_w;
+  c->tuned[key] = {best_w};
+  return PYIID_OK;
+}
+
+int tune_fused(pyiid_ctx* c, int slot, double qbin) {
+  Table& t = c->tab[slot];
+  const std::vector<int64_t> key{0, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
+                                 static_cast<int64_t>(c->struct_stamp)};
+  auto hit = c->tuned.find(key);
+  if (hit != c->tuned.end()) {
+    c->grad_m = static_cast<int>(hit->second[0]);
+    c->fused_waves = hit->second[1];
+    return PYIID_OK;
+  }
+  if (c->capturing || !c->autotune) return PYIID_OK;
+  double best = 1e300;
+  int best_m = c->grad_m;
+  int64_t best_w = c->fused_waves;
+  for (int m : {32, 16})
+    for (int64_t w : {16384LL, 32768LL, 65536LL}) {
+      c->grad_m = m;
+      c->fused_waves = w;
+      double ms = 0.0;
+      CHK(run_fused(c, slot, qbin, nullptr, false));
+      CHK(time_launch(c, [&]() { return run_fused(c, slot, qbin, nullptr, false); }, &ms));
+      if (ms < best) {
+        best = ms;
+        best_m = m;
+        best_w = w;
+      }
+    }
+  c->grad_m = best_m;
+  c->fused_waves = best_w;
+  c->tuned[key] = {best_m, best_w};
+  return PYIID_OK;
+}
+
+// forces (n x 3 doubles) = scale * sum_q u[chain][q] grad(i, c, q): the
+// unordered-pair kernel when its tables fit in LDS, else the fused kernel.
+int contract_forces(pyiid_ctx* c, int slot, double qbin, double scale, double* out) {
+  Table& t = c->tab[slot];
+  const int nq = t.nq, B = c->n_chain;
+  CHK(ensure_packed(c));
+  CHK(ensure_norm(c, slot));
+  const bool pair_ok = !weighted_table(t) ||
+                       sizeof(float) * (static_cast<size_t>(t.n_elem) * nq + 2 * kWaves * 32) <=
+                           kPairForceLds;
+  if (pair_ok) {
+    CHK(c->uq.reserve(sizeof(float) * static_cast<size_t>(std::max(1, nq)) * B));
+    HIPCHK(launch_uq(c->stream, c->u.as<float>(), B, nq,
+                     static_cast<double>(static_cast<float>(qbin)), c->uq.as<float>()));
+    CHK(tune_contract(c, slot, qbin));
+    Timed tm(c, 4);
+    CHK(launch_contract(c, slot, qbin));
+    CHK(tm.done());
+    HIPCHK(launch_reduce_pair_force(c->stream, c->pfpart.as<float>(), c->pair_entries.as<int>(),
+                                    c->pair_items.ranges.as<Range>(), c->pair_items.n_ranges,
+                                    scale, out));
+    return PYIID_OK;
+  }
+  CHK(run_fused(c, slot, qbin, c->u.as<float>()));
+  const int n_qchunk = ((nq + kWaves * c->grad_m - 1) / (kWaves * c->grad_m)) * kWaves;
+  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
+                             c->fused_items.n_ranges, n_qchunk, scale, out));
   return PYIID_OK;
 }
 
@@ -510,11 +718,7 @@ int energy_forces_device(pyiid_ctx* c, int slot, double qbin, int potential,
   CHK(c->u.reserve(sizeof(float) * (size_t)nq * B));
   HIPCHK(launch_adjoint(c->stream, c->tmat.as<float>(), nr, nq, c->w.as<double>(),
                         c->tab[slot].gs.as<double>(), B, c->u.as<float>()));
-  CHK(run_fused(c, slot, qbin, c->u.as<float>()));
-  const int n_qchunk = ((nq + kWaves * kGradM - 1) / (kWaves * kGradM)) * kWaves;
-  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
-                             c->fused_items.n_ranges, n_qchunk, force_scale, forces_dev));
-  return PYIID_OK;
+  return contract_forces(c, slot, qbin, force_scale, forces_dev);
 }
 
 int check_pdf_setup(pyiid_ctx* c, int slot, int potential) {
@@ -1007,7 +1211,8 @@ int pyiid_destroy(pyiid_ctx* c) {
                  &c->sur.d, &c->sur.cbuf, &c->sur.eatom, &c->sur.fsur, &c->sur.phi,
                  &c->sur.gpart, &c->sur.gout, &c->sur.viol, &c->sur.viol_count,
                  &c->sur.atom_chain, &c->sur.tiles64, &c->sur.tiles256, &c->sur.chunks,
-                 &c->sur.granges})
+                 &c->sur.granges, &c->pair_items.items, &c->pair_items.ranges,
+                 &c->pair_entries, &c->uq, &c->pfpart})
     b->release();
   for (Table& t : c->tab) {
     t.tab.release();
@@ -1043,7 +1248,7 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
   CHK(check_ctx(c));
   CHK(resolve_timing(c));
   c->timing = enable != 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < 5; ++k) {
     c->t_ms[k] = 0.0;
     c->t_n[k] = 0;
   }
@@ -1052,7 +1257,7 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
 
 int pyiid_kernel_time(pyiid_ctx* c, int kind, double* total_ms, int64_t* launches) {
   CHK(check_ctx(c));
-  if (kind < 0 || kind > 3) return fail(PYIID_E_INVALID, "kernel kind %d not in [0, 3]", kind);
+  if (kind < 0 || kind > 4) return fail(PYIID_E_INVALID, "kernel kind %d not in [0, 4]", kind);
   CHK(resolve_timing(c));
   if (total_ms) *total_ms = c->t_ms[kind];
   if (launches) *launches = c->t_n[kind];
@@ -1445,12 +1650,9 @@ int pyiid_fq_energy_forces(pyiid_ctx* c, int slot, double qbin, int32_t q_first,
   CHK(c->u.reserve(sizeof(float) * nq));
   HIPCHK(launch_adjoint_fq(c->stream, nq, q_first, c->w.as<double>(), c->tab[slot].gs.as<double>(),
                            c->u.as<float>()));
-  CHK(run_fused(c, slot, qbin, c->u.as<float>()));
   CHK(c->io_c.reserve(sizeof(double) * 3 * c->n));
   double* fdst = (flags & PYIID_DEVICE) ? forces : c->io_c.as<double>();
-  const int n_qchunk = ((nq + kWaves * kGradM - 1) / (kWaves * kGradM)) * kWaves;
-  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
-                             c->fused_items.n_ranges, n_qchunk, conv, fdst));
+  CHK(contract_forces(c, slot, qbin, conv, fdst));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, forces, fdst, sizeof(double) * 3 * c->n, flags));
   return finish(c, flags);
 }

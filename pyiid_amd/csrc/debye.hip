@@ -368,6 +368,150 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
   if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine;
 }
 
+// ---------------------------------------------------------------------------
+// Contracted forces over unordered pairs.  An item is a pair of 64-atom tiles
+// (I, J), J <= I, of one chain; a wave takes one item and one Q group.  Lane l
+// holds atom I_l; the J tile's atoms and their force accumulators turn round
+// the wave one lane per step, so lane l meets J_((l+s) mod 64) at step s and
+// every unordered pair is met once (a diagonal item, I = J, runs steps 1..32
+// and drops the duplicate half of step 32).  Per pair the Q sum collapses to
+// two scalars
+//   A = sum_q u_q Q_q ff cos(Q r),   B = sum_q u_q ff sin(Q r),
+// and the pair adds S (q_i - q_j) / r^2 to atom i and the negative to atom j
+// with S = A - B / r: the same quantity as sum_q u_q grad(i, c, q) of the
+// fused kernel (t = Q cos - sin / r there), without per-bin accumulators.
+//   part[work][side][64][3]  (side 0: I tile, 1: J tile; work = item * groups + g)
+// The phasor is re-anchored from an exact sincos every kPfChunk bins.
+// ---------------------------------------------------------------------------
+constexpr int kPfChunk = 32;
+
+template <bool WEIGHTED>
+__global__ __launch_bounds__(256) void pair_force_kernel(
+    const AtomRec* __restrict__ pos, const float* __restrict__ ftab, int n_elem, int nq,
+    double dq, const Item* __restrict__ items, int n_work, int n_groups, int span,
+    const float* __restrict__ u, const float* __restrict__ uq, float* __restrict__ part) {
+  extern __shared__ __attribute__((aligned(16))) float lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int work = blockIdx.x * kWaves + wave;
+  const bool active = work < n_work;
+  const int item = active ? work / n_groups : 0;
+  const int g = active ? work - item * n_groups : 0;
+  const Item it = items[item];
+  const int q_lo = g * span;
+  float2* uw = reinterpret_cast<float2*>(lds) + static_cast<size_t>(wave) * span;
+  const float* uc = u + static_cast<size_t>(it.chain) * nq;
+  const float* uqc = uq + static_cast<size_t>(it.chain) * nq;
+  for (int k = lane; k < span; k += 64) {
+    const int q = q_lo + k;
+    uw[k] = (active && q < nq) ? make_float2(uc[q], uqc[q]) : make_float2(0.f, 0.f);
+  }
+  float* tab = lds + 2 * kWaves * span;
+  if constexpr (WEIGHTED) {
+    for (int t = threadIdx.x; t < n_elem * nq; t += blockDim.x) tab[t] = ftab[t];
+  }
+  __syncthreads();
+  if (!active) return;
+
+  const bool diag = it.jbeg == it.i_base;
+  const int i = it.i_base + lane;
+  const bool i_ok = i < it.i_end;
+  const AtomRec pi = pos[i_ok ? i : it.i_base];
+  int jl = lane;  // slot (atom jbeg + jl) whose data and accumulators this lane holds
+  AtomRec pj = pos[it.jbeg + lane < it.jend ? it.jbeg + lane : it.jbeg];
+  float fix = 0.f, fiy = 0.f, fiz = 0.f, fjx = 0.f, fjy = 0.f, fjz = 0.f;
+  const int src = (lane + 1) & 63;
+  auto turn = [&]() {
+    pj.x = __shfl(pj.x, src, 64);
+    pj.y = __shfl(pj.y, src, 64);
+    pj.z = __shfl(pj.z, src, 64);
+    pj.w = __shfl(pj.w, src, 64);
+    jl = __shfl(jl, src, 64);
+    fjx = __shfl(fjx, src, 64);
+    fjy = __shfl(fjy, src, 64);
+    fjz = __shfl(fjz, src, 64);
+  };
+  const int s_begin = diag ? 1 : 0, s_end = diag ? 33 : 64;
+  if (diag) turn();
+  const float* ti = tab;
+  if constexpr (WEIGHTED) ti = tab + static_cast<int>(pi.w) * nq;
+  for (int s = s_begin; s < s_end; ++s) {
+    const bool j_ok = it.jbeg + jl < it.jend;
+    const bool keep = i_ok && j_ok && !(diag && s == 32 && lane >= 32);
+    const PairGeom pg = pair_geom(pi, pj, keep);
+    const double dth = dq * pg.r;
+    float s1, c1;
+    sincos_red(dth, s1, c1);
+    const float* tj = tab;
+    if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
+    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    for (int c0 = 0; c0 < span; c0 += kPfChunk) {
+      float sn, cs;
+      sincos_red(static_cast<double>(q_lo + c0) * dth, sn, cs);
+#pragma unroll
+      for (int m = 0; m < kPfChunk; ++m) {
+        const float2 w = uw[c0 + m];
+        float wu = w.x, wq = w.y;
+        if constexpr (WEIGHTED) {
+          const int q = min(q_lo + c0 + m, nq - 1);
+          const float ff = ti[q] * tj[q];
+          wu *= ff;
+          wq *= ff;
+        }
+        if (m & 1) {
+          a1 = fmaf(wq, cs, a1);
+          b1 = fmaf(wu, sn, b1);
+        } else {
+          a0 = fmaf(wq, cs, a0);
+          b0 = fmaf(wu, sn, b0);
+        }
+        const float sn2 = fmaf(sn, c1, cs * s1);
+        cs = fmaf(cs, c1, -sn * s1);
+        sn = sn2;
+      }
+    }
+    const float S = (a0 + a1) - (b0 + b1) * pg.inv_r;
+    fix = fmaf(S, pg.dpx, fix);
+    fiy = fmaf(S, pg.dpy, fiy);
+    fiz = fmaf(S, pg.dpz, fiz);
+    fjx = fmaf(-S, pg.dpx, fjx);
+    fjy = fmaf(-S, pg.dpy, fjy);
+    fjz = fmaf(-S, pg.dpz, fjz);
+    turn();
+  }
+  float* o = part + static_cast<size_t>(work) * 384;
+  o[lane * 3 + 0] = fix;
+  o[lane * 3 + 1] = fiy;
+  o[lane * 3 + 2] = fiz;
+  o[192 + jl * 3 + 0] = fjx;
+  o[192 + jl * 3 + 1] = fjy;
+  o[192 + jl * 3 + 2] = fjz;
+}
+
+// force[i][c] = conv * sum over the tile's (work, side) entries (FP64, fixed
+// order); one thread per (tile, lane, component).
+__global__ void reduce_pair_force_kernel(const float* __restrict__ part,
+                                         const int* __restrict__ entries,
+                                         const Range* __restrict__ tiles, int n_tiles,
+                                         double conv, double* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_tiles * 192) return;
+  const int tile = static_cast<int>(t / 192), rem = static_cast<int>(t % 192);
+  const Range tr = tiles[tile];
+  if (tr.i_base + rem / 3 >= tr.i_end) return;
+  double acc = 0.0;
+  for (int e = tr.begin; e < tr.end; ++e)
+    acc += static_cast<double>(part[static_cast<size_t>(entries[e]) * 192 + rem]);
+  out[static_cast<size_t>(tr.i_base) * 3 + rem] = acc * conv;
+}
+
+// uq[b][q] = u[b][q] * Q_q, Q_q = float32(q * dq) as in the Debye kernels
+__global__ void uq_kernel(const float* __restrict__ u, int64_t total, int nq, double dq,
+                          float* __restrict__ uq) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e < total) uq[e] = u[e] * static_cast<float>(static_cast<double>(e % nq) * dq);
+}
+
 __global__ void reduce_ranges_kernel(const double* __restrict__ part,
                                      const Range* __restrict__ ranges, int n_rows, int nq,
                                      const double* __restrict__ scale, double* __restrict__ out64,
@@ -477,12 +621,11 @@ int grid_for(int64_t total, int block) {
 
 }  // namespace
 
-hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
-                             int n_elem, int nq, double dq, const Item* items, int n_items,
-                             bool weighted, float* gpart, double* fpart, const float* u,
-                             float* force_part) {
-  constexpr int M = kGradM;
-  if (n_items <= 0) return hipSuccess;
+template <int M>
+hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
+                          int n_elem, int nq, double dq, const Item* items, int n_items,
+                          bool weighted, float* gpart, double* fpart, const float* u,
+                          float* force_part) {
   dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
   dim3 block(256);
   const bool contract = (u != nullptr);
@@ -501,6 +644,56 @@ hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n_total, co
       hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, n_total,
                          ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, int n_total,
+                             const float* ftab, int n_elem, int nq, double dq, const Item* items,
+                             int n_items, bool weighted, float* gpart, double* fpart,
+                             const float* u, float* force_part) {
+  if (n_items <= 0) return hipSuccess;
+  if (m == 16)
+    return launch_fused_m<16>(st, pos, n_total, ftab, n_elem, nq, dq, items, n_items, weighted,
+                              gpart, fpart, u, force_part);
+  if (m == 32)
+    return launch_fused_m<32>(st, pos, n_total, ftab, n_elem, nq, dq, items, n_items, weighted,
+                              gpart, fpart, u, force_part);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_pair_force(hipStream_t st, const AtomRec* pos, const float* ftab, int n_elem,
+                             int nq, double dq, const Item* items, int n_items, int n_groups,
+                             int span, bool weighted, const float* u, const float* uq,
+                             float* part) {
+  const int n_work = n_items * n_groups;
+  if (n_work <= 0) return hipSuccess;
+  const size_t lds = sizeof(float) * (2 * kWaves * static_cast<size_t>(span) +
+                                      (weighted ? static_cast<size_t>(n_elem) * nq : 0));
+  if (span % kPfChunk != 0 || lds > kPairForceLds) return hipErrorInvalidValue;
+  dim3 grid((n_work + kWaves - 1) / kWaves);
+  if (weighted)
+    hipLaunchKernelGGL(pair_force_kernel<true>, grid, dim3(256), lds, st, pos, ftab, n_elem, nq,
+                       dq, items, n_work, n_groups, span, u, uq, part);
+  else
+    hipLaunchKernelGGL(pair_force_kernel<false>, grid, dim3(256), lds, st, pos, ftab, n_elem, nq,
+                       dq, items, n_work, n_groups, span, u, uq, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int* entries,
+                                    const Range* tiles, int n_tiles, double conv, double* out) {
+  const int64_t total = (int64_t)n_tiles * 192;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(reduce_pair_force_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
+                     dim3(256), 0, st, part, entries, tiles, n_tiles, conv, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq) {
+  const int64_t total = (int64_t)n_chain * nq;
+  if (total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(uq_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0,
+                     st, u, total, nq, dq, uq);
   return hipGetLastError();
 }
 

@@ -36,13 +36,29 @@ struct Range {
 };
 
 // ---- launch wrappers (debye.hip) ----
-// Ordered-pair F + gradient.  Non-contract: gpart[item.split][i][c][q] (float)
-// for i in the item's tile.  Contract (u != null): force_part[item][qchunk][64][3]
-// = sum_q u[chain][q] * grad(i, c, q).  fpart[item][q]: F partial (double).
-hipError_t launch_fused_grad(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
-                             int n_elem, int nq, double dq, const Item* items, int n_items,
-                             bool weighted, float* gpart, double* fpart, const float* u,
-                             float* force_part);
+// Ordered-pair F + gradient, m = 16 or 32 Q bins per wave.  Non-contract:
+// gpart[item.split][i][c][q] (float) for i in the item's tile.  Contract
+// (u != null): force_part[item][qchunk][64][3] = sum_q u[chain][q] * grad(i, c, q),
+// qchunk over ceil(nq / (kWaves m)) * kWaves.  fpart[item][q]: F partial (double).
+hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, int n_total,
+                             const float* ftab, int n_elem, int nq, double dq, const Item* items,
+                             int n_items, bool weighted, float* gpart, double* fpart,
+                             const float* u, float* force_part);
+// Contracted forces over unordered pairs: items are 64-atom tile pairs (I, J),
+// J <= I, of one chain (Item.tile = I's and Item.pad = J's chain-global tile
+// index); n_groups Q groups of `span` bins (a multiple of 32) each.
+// part[(item * n_groups + g)][side][64][3], side 0 = I, 1 = J.
+constexpr size_t kPairForceLds = 64 * 1024;
+hipError_t launch_pair_force(hipStream_t st, const AtomRec* pos, const float* ftab, int n_elem,
+                             int nq, double dq, const Item* items, int n_items, int n_groups,
+                             int span, bool weighted, const float* u, const float* uq,
+                             float* part);
+// force[i][c] = conv * sum_{e in entries[tile]} part[e][lane][c] (FP64), where
+// an entry is work * 2 + side; tiles[t] = {entry begin, end, i_base, i_end}.
+hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int* entries,
+                                    const Range* tiles, int n_tiles, double conv, double* out);
+// uq[b][q] = u[b][q] * float32(q dq)
+hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
 // Unordered-pair F only: fpart[item][q] (double).
 hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
                      int n_elem, int nq, double dq, const Item* items, int n_items,
