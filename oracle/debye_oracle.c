@@ -154,3 +154,99 @@ int oracle_flat_f32(const float *pos, const float *scat, int64_t n, int nq, floa
   free(th);
   return 0;
 }
+
+/*
+ * FP64 sums on a subset of Q bins (and of gradient rows), for checks at sizes
+ * where the numpy oracle is too slow.  Same mathematics as oracle/scatter.py
+ * fq / grad_fq (the FP64 restatement of cpu_flat.py:77-153):
+ *   fsum[a]          = sum_{i>j} f_i(Q_a) f_j(Q_a) sin(Q_a r_ij) / r_ij
+ *   gsum[row][c][a]  = sum_{j != i} f_i f_j (Q_a cos(Q_a r) - sin(Q_a r)/r) / r^2 (q_i - q_j)_c
+ * with i = rows[row]; the caller applies 2/na and -1/na.  f_e(Q_a) comes from
+ * ftab[e][a] (n_elem x nqs) through elem[i].
+ */
+typedef struct {
+  const double *pos, *ftab, *Q;
+  const int *elem, *rows;
+  int64_t n, k0, k1;
+  int nqs, r0, r1;
+  double *fsum; /* private, nqs */
+  double *gsum; /* shared, rows [r0, r1) owned by this task */
+} task64_t;
+
+static void *worker64(void *arg) {
+  task64_t *t = (task64_t *)arg;
+  const int nqs = t->nqs;
+  int64_t i, j;
+  if (t->k1 > t->k0) {
+    k_to_ij(t->k0, &i, &j);
+    for (int64_t k = t->k0; k < t->k1; ++k) {
+      const double dx = t->pos[3 * i] - t->pos[3 * j], dy = t->pos[3 * i + 1] - t->pos[3 * j + 1],
+                   dz = t->pos[3 * i + 2] - t->pos[3 * j + 2];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      const double *fi = t->ftab + (size_t)t->elem[i] * nqs;
+      const double *fj = t->ftab + (size_t)t->elem[j] * nqs;
+      for (int a = 0; a < nqs; ++a) t->fsum[a] += fi[a] * fj[a] * sin(t->Q[a] * r) / r;
+      if (++j == i) {
+        ++i;
+        j = 0;
+      }
+    }
+  }
+  for (int row = t->r0; row < t->r1; ++row) {
+    const int64_t ii = t->rows[row];
+    double *g = t->gsum + (size_t)row * 3 * nqs;
+    const double *fi = t->ftab + (size_t)t->elem[ii] * nqs;
+    for (int64_t jj = 0; jj < t->n; ++jj) {
+      if (jj == ii) continue;
+      const double d[3] = {t->pos[3 * ii] - t->pos[3 * jj], t->pos[3 * ii + 1] - t->pos[3 * jj + 1],
+                           t->pos[3 * ii + 2] - t->pos[3 * jj + 2]};
+      const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      const double r = sqrt(r2);
+      const double *fj = t->ftab + (size_t)t->elem[jj] * nqs;
+      for (int a = 0; a < nqs; ++a) {
+        const double qr = t->Q[a] * r;
+        const double w = fi[a] * fj[a] * (t->Q[a] * cos(qr) - sin(qr) / r) / r2;
+        for (int c = 0; c < 3; ++c) g[(size_t)c * nqs + a] += w * d[c];
+      }
+    }
+  }
+  return NULL;
+}
+
+int oracle_sums_f64(const double *pos, const int *elem, const double *ftab, const double *Q,
+                    int64_t n, int nqs, const int *rows, int nrows, int nthreads, double *fsum,
+                    double *gsum) {
+  if (n < 0 || nqs <= 0 || nthreads < 1 || nrows < 0) return -1;
+  const int64_t kmax = n * (n - 1) / 2;
+  task64_t *tasks = (task64_t *)calloc((size_t)nthreads, sizeof(task64_t));
+  pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+  double *priv = (double *)calloc((size_t)nthreads * nqs, sizeof(double));
+  if (!tasks || !th || !priv) return -2;
+  memset(fsum, 0, sizeof(double) * nqs);
+  if (gsum) memset(gsum, 0, sizeof(double) * (size_t)nrows * 3 * nqs);
+  for (int t = 0; t < nthreads; ++t) {
+    task64_t *k = &tasks[t];
+    k->pos = pos;
+    k->ftab = ftab;
+    k->Q = Q;
+    k->elem = elem;
+    k->rows = rows;
+    k->n = n;
+    k->nqs = nqs;
+    k->k0 = kmax * t / nthreads;
+    k->k1 = kmax * (t + 1) / nthreads;
+    k->r0 = gsum ? nrows * t / nthreads : 0;
+    k->r1 = gsum ? nrows * (t + 1) / nthreads : 0;
+    k->fsum = priv + (size_t)t * nqs;
+    k->gsum = gsum;
+    pthread_create(&th[t], NULL, worker64, k);
+  }
+  for (int t = 0; t < nthreads; ++t) {
+    pthread_join(th[t], NULL);
+    for (int a = 0; a < nqs; ++a) fsum[a] += priv[(size_t)t * nqs + a];
+  }
+  free(priv);
+  free(th);
+  free(tasks);
+  return 0;
+}

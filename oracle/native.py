@@ -21,6 +21,11 @@ def load():
                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                         ctypes.c_void_p]
         lib.oracle_flat_f32.restype = ctypes.c_int
+        lib.oracle_sums_f64.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_int,
+                                                                 ctypes.c_void_p, ctypes.c_int,
+                                                                 ctypes.c_int, ctypes.c_void_p,
+                                                                 ctypes.c_void_p]
+        lib.oracle_sums_f64.restype = ctypes.c_int
         _lib = lib
     return _lib
 
@@ -41,3 +46,38 @@ def flat_f32(q, scatter, qbin, q_index=None, nthreads=1, grad=True):
     if rc != 0:
         raise RuntimeError("oracle_flat_f32 failed")
     return fq, g
+
+
+def fp64_subset(q, elem_index, table, qbin, q_index, rows=None, nthreads=1):
+    """FP64 F(Q) and -1/2 dF/dq (oracle/scatter.py fq / grad_fq) on the bins
+    ``q_index`` and gradient rows ``rows`` (None: no gradient), threaded C.
+    ``table`` is the per-element form factor table (E x Qb), ``elem_index``
+    the element row of every atom; positions are taken as float32 (the
+    reference's cast)."""
+    from . import scatter as S
+    q = np.ascontiguousarray(np.asarray(q, np.float32), dtype=np.float64)
+    elem = np.ascontiguousarray(elem_index, dtype=np.int32)
+    qi = np.asarray(q_index)
+    ft = np.asarray(table, np.float32).astype(np.float64)
+    fsel = np.ascontiguousarray(ft[:, qi])
+    Q = np.ascontiguousarray(S.q_values(qbin, ft.shape[1])[qi])
+    n = len(q)
+    rw = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+    fsum = np.empty(len(qi))
+    gsum = None if rw is None else np.empty((len(rw), 3, len(qi)))
+    rc = load().oracle_sums_f64(q.ctypes.data, elem.ctypes.data, fsel.ctypes.data, Q.ctypes.data,
+                                n, len(qi), None if rw is None else rw.ctypes.data,
+                                0 if rw is None else len(rw), int(nthreads), fsum.ctypes.data,
+                                None if gsum is None else gsum.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("oracle_sums_f64 failed")
+    # na = ((sum_e c_e f_e)^2 - sum_e c_e f_e^2) / (N - 1)  (closed form of N * mean f_i f_j)
+    counts = np.bincount(elem, minlength=ft.shape[0]).astype(np.float64)
+    s1 = counts @ fsel
+    s2 = counts @ (fsel * fsel)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        na = (s1 * s1 - s2) / (n - 1) if n > 1 else np.zeros_like(s1)
+        fq = np.nan_to_num(2.0 * fsum / na, nan=0.0, posinf=0.0, neginf=0.0)
+        grad = None if gsum is None else np.nan_to_num(-gsum / na, nan=0.0, posinf=0.0,
+                                                        neginf=0.0)
+    return fq, grad

@@ -54,3 +54,15 @@ def random_alloy(n, z_a, z_b, seed, fraction_a=0.5):
     na = int(round(n * fraction_a))
     numbers = np.array([z_a] * na + [z_b] * (n - na))
     return numbers[rs.permutation(n)]
+
+
+def octahedron(length, a=4.08):
+    """Regular FCC octahedron with ``length`` atoms on every edge (the
+    ``ase.cluster.Octahedron(symbol, length)`` construction without cutoff):
+    sites (i, j, k) a/2 with |i| + |j| + |k| <= length - 1 and i + j + k of
+    the parity of length - 1; 6, 19, 44, 85, ... atoms."""
+    L = int(length) - 1
+    r = np.arange(-L, L + 1)
+    i, j, k = np.meshgrid(r, r, r, indexing="ij")
+    keep = (np.abs(i) + np.abs(j) + np.abs(k) <= L) & ((i + j + k - L) % 2 == 0)
+    return np.stack([i[keep], j[keep], k[keep]], axis=1).astype(np.float64) * (a / 2.0)

@@ -161,3 +161,31 @@ def test_calc1d_alloy_forces_vs_oracle():
     gp = S.grad_pdf(gfq, e_["rstep"], pq, r, e_["qmin"])
     assert abs(e - 100 * S.get_rw(target, gcalc)[0]) < 1e-4
     assert rel(fo, 100 * S.wrap_grad_rw(gp, gcalc, target)) < TOL
+
+
+def test_nuts_example_runs_on_device():
+    # examples/Au_NP_PDF.py: an Au octahedron dilated by 5 % refined against
+    # the PDF of the original with the NUTS ensemble and Calc1D(conv=100, 'rw')
+    from pyiid_amd.nuts import NUTSCanonicalEnsemble
+    atoms = Atoms(numbers=np.full(19, 79), positions=structures.octahedron(3))
+    scat = ElasticScatter(seed=0)
+    pdf = scat.get_pdf(atoms)
+    atoms2 = atoms.copy()
+    atoms2.positions *= 1.05
+    calc = Calc1D(target_data=pdf, exp_function=scat.get_pdf, exp_grad_function=scat.get_grad_pdf,
+                  conv=100, potential="rw")
+    atoms2.set_calculator(calc)
+    e_start = atoms2.get_potential_energy()
+    ens = NUTSCanonicalEnsemble(atoms2, temperature=1000, escape_level=5, seed=3)
+    traj, meta = ens.run(4)
+    assert len(traj) == 1 + meta["accepted_samples"] and meta["samples_total"] > 0
+    last = traj[-1]
+    e_last = last.get_potential_energy()
+    assert np.isfinite(e_start) and np.isfinite(e_last)
+    # the device energy of the final state against the oracle's Rw
+    e_, pq = S.make_experiment({})
+    r = S.r_grid(e_)
+    scat._wrap_atoms(last)
+    sp = last.get_array("PDF scatter")
+    gcalc = S.get_pdf_at_qmin(S.fq(last.positions, sp, pq), e_["rstep"], pq, r, e_["qmin"])
+    assert abs(e_last - 100 * S.get_rw(pdf, gcalc)[0]) < 1e-4

@@ -42,3 +42,19 @@ def test_golden_fixture_regression():
     gr = S.get_pdf_at_qmin(S.fq(d["positions"], sp, pq), exp["rstep"], pq, r, exp["qmin"])
     np.testing.assert_allclose(gr, d["gr"], rtol=1e-12, atol=1e-12)
     assert abs(S.get_rw(d["gr_ideal"], gr)[0] - float(d["rw"])) < 1e-12
+
+
+def test_fp64_subset_matches_numpy_oracle():
+    from oracle import native, scatter as S
+    from pyiid_amd import structures
+    from pyiid_amd.formfactors import element_table
+    for numbers in (structures.random_alloy(230, 78, 28, seed=1), np.full(230, 79)):
+        pos = structures.jitter(structures.fcc_nearest(230, a=3.724), 0.05, seed=2)
+        table, index, _ = element_table(numbers, 0.05, 400)
+        qi = np.array([0, 3, 77, 250, 399])
+        rows = np.array([0, 17, 229])
+        fq, g = native.fp64_subset(pos, index, table, 0.05, qi, rows, nthreads=4)
+        sp = table[index]
+        np.testing.assert_allclose(fq, S.fq(pos, sp, 0.05, q_index=qi), rtol=1e-11, atol=1e-13)
+        want = S.grad_fq(pos, sp, 0.05, q_index=qi, atom_index=rows)
+        np.testing.assert_allclose(g, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
