@@ -172,9 +172,14 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
   const bool lane_ok = i < it.i_end;
   const AtomRec pi = pos[lane_ok ? i : it.i_base];
 
+  // Q values stay in VGPRs: an FMA with a scalar-register operand issues
+  // more slowly than an all-VGPR one on gfx950.
   float qv[M];
 #pragma unroll
-  for (int m = 0; m < M; ++m) qv[m] = uniform_f(static_cast<float>(static_cast<double>(q0 + m) * dq));
+  for (int m = 0; m < M; ++m) {
+    qv[m] = static_cast<float>(static_cast<double>(q0 + m) * dq);
+    asm volatile("" : "+v"(qv[m]));
+  }
 
   float fi[WEIGHTED ? M : 1];
   if constexpr (WEIGHTED) {
@@ -206,8 +211,13 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
       float sn, cs, s1, c1;
       sincos_red(q0d * dth, sn, cs);
       sincos_red(dth, s1, c1);
-      // Along Q the phasor (cs, sn) turns by dtheta: exact rotation (four
-      // FMA-class ops), accurate for every dtheta including close pairs.
+      // Along Q the phasor turns by dtheta (exact rotation, four FMA-class
+      // ops, accurate for every dtheta including close pairs).  It is carried
+      // as (w, cs) with w = sin(Q r) / r, so omega needs no multiply:
+      //   w' = w c1 + cs (s1 / r),  cs' = cs c1 - w (s1 r).
+      const float s1_r = s1 * g.inv_r;
+      const float s1r = s1 * static_cast<float>(g.r);
+      float w = sn * g.inv_r;
       if constexpr (WEIGHTED) {
         const int ej = static_cast<int>(pj.w);
         const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
@@ -218,29 +228,27 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
           for (int e = 0; e < 4; ++e) {
             const int m = m4 * 4 + e;
             const float ff = fi[m] * fj[e];
-            const float w = sn * g.inv_r;
             const float t = fmaf(qv[m], cs, -w) * ff;
             aF[m] = fmaf(w, ff, aF[m]);
             ax[m] = fmaf(t, g.dpx, ax[m]);
             ay[m] = fmaf(t, g.dpy, ay[m]);
             az[m] = fmaf(t, g.dpz, az[m]);
-            const float sn2 = fmaf(sn, c1, cs * s1);
-            cs = fmaf(cs, c1, -sn * s1);
-            sn = sn2;
+            const float w2 = fmaf(w, c1, cs * s1_r);
+            cs = fmaf(cs, c1, -w * s1r);
+            w = w2;
           }
         }
       } else {
 #pragma unroll
         for (int m = 0; m < M; ++m) {
-          const float w = sn * g.inv_r;
           const float t = fmaf(qv[m], cs, -w);
           aF[m] += w;
           ax[m] = fmaf(t, g.dpx, ax[m]);
           ay[m] = fmaf(t, g.dpy, ay[m]);
           az[m] = fmaf(t, g.dpz, az[m]);
-          const float sn2 = fmaf(sn, c1, cs * s1);
-          cs = fmaf(cs, c1, -sn * s1);
-          sn = sn2;
+          const float w2 = fmaf(w, c1, cs * s1_r);
+          cs = fmaf(cs, c1, -w * s1r);
+          w = w2;
         }
       }
     }
