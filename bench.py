@@ -161,12 +161,89 @@ def batched_hmc(n_prop=3, seed=0):
             "surrogate_s_per_proposal": float(out["times_ms"][-1]) / 1e3 / (4 * n_prop)}
 
 
+def kernel_suite(reps=3):
+    """Each hot kernel on its own workload, one warm-up and `reps` timed
+    launches (HIP events on the context stream), for profiling runs:
+    F-only and pair stage on the 6,699-atom sphere (1,250 bins; all 22.4 M
+    pairs), the transform GEMM on its grad-G rows (3 N x 330 -> 4,000 r
+    points), contracted forces and learned-integrator forces on 64 chains of
+    5,000 atoms."""
+    import torch
+    from pyiid_amd import _lib, workloads
+    from pyiid_amd.elasticscatter import ElasticScatter
+    from pyiid_amd.sim import SurrogateHMC
+    dev = _lib.PYIID_DEVICE
+    out = {}
+    pos, table, index = workload()
+    n = len(pos)
+    pairs = n * (n - 1) // 2
+
+    def timed(ctx, kind, fn):
+        fn()
+        ctx.synchronize()
+        ctx.enable_timing(True)
+        for _ in range(reps):
+            fn()
+        ms, cnt = ctx.kernel_time(kind)
+        ctx.enable_timing(False)
+        return ms / max(1, cnt)
+
+    ctx = _lib.Context(0)
+    ctx.set_positions(pos)
+    ctx.set_elements(index)
+    ctx.set_scatter(_lib.SLOT_FQ, table)
+    fq = torch.empty(NQ, dtype=torch.float32, device="cuda")
+    ms = timed(ctx, _lib.KERNEL_FQ, lambda: ctx.fq(_lib.SLOT_FQ, QBIN, NQ, out=fq.data_ptr(),
+                                                   flags=dev))
+    out["fq_kernel"] = {"ms": ms, "pair_q_per_s": pairs * NQ / (ms * 1e-3)}
+    d = torch.empty((pairs, 3), dtype=torch.float32, device="cuda")
+    r = torch.empty(pairs, dtype=torch.float32, device="cuda")
+    ms = timed(ctx, _lib.KERNEL_PAIRS, lambda: _lib.check(ctx.lib.pyiid_pair_geometry(
+        ctx.handle, 0, pairs, d.data_ptr(), r.data_ptr(), dev)))
+    out["pair_geometry_kernel"] = {"ms": ms, "bytes": 16 * pairs + 32 * n,
+                                   "gb_per_s": (16 * pairs + 32 * n) / (ms * 1e-3) / 1e9}
+    del d, r
+    scat = ElasticScatter(dict(workloads.EXP))
+    nqp = int(np.floor(scat.exp["qmax"] / scat.pdf_qbin))
+    rgrid = scat.get_r()
+    ctx.set_transform(scat.exp["rstep"], scat.pdf_qbin, rgrid, scat.exp["qmin"], nqp)
+    gq = torch.randn((n, 3, nqp), dtype=torch.float32, device="cuda")
+    gg = torch.empty((n, 3, len(rgrid)), dtype=torch.float64, device="cuda")
+    ms = timed(ctx, _lib.KERNEL_GEMM, lambda: ctx.grad_pdf(gq, len(rgrid), out=gg, flags=dev))
+    flop = 2.0 * 3 * n * nqp * len(rgrid)
+    out["gemm_nt_kernel"] = {"ms": ms, "rows": 3 * n, "k": nqp, "n": len(rgrid),
+                             "tflops": flop / (ms * 1e-3) / 1e12}
+    del gq, gg
+    ctx.close()
+
+    starts, targs, numbers = workloads.build("large", 0)
+    gt = workloads.target_pdfs(scat, targs, numbers)
+    h = SurrogateHMC(scat, list(zip(starts, numbers)), gt, conv=100.0, masses="unit")
+    nat = int(h.offsets[-1])
+    ms = timed(h.ctx, _lib.KERNEL_PAIR_FORCE, lambda: h.ctx.pdf_energy_forces(
+        _lib.SLOT_PDF, scat.pdf_qbin, gt, _lib.POT_RW, 100.0, nat))
+    kpairs = sum(len(x) * (len(x) - 1) // 2 for x in starts)
+    out["pair_force_kernel"] = {"ms": ms, "pair_q_per_s": kpairs * nqp / (ms * 1e-3)}
+    h.train()
+    h.forces()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        h.ctx.sur_forces(nat)
+    out["surrogate_forces_call"] = {"ms": (time.perf_counter() - t0) / reps * 1e3,
+                                    "atoms": nat}
+    h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-suite", action="store_true",
+                    help="only run every hot kernel on its own workload (for profiling)")
     ap.add_argument("--hmc-proposals", type=int, default=16,
                     help="exact-HMC proposals of the sampler comparison (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -181,6 +258,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print("copied 3 x %d bytes" % (4 * src.numel()))
+        return
+    if args.kernel_suite:
+        print(json.dumps(kernel_suite()))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

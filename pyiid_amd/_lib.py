@@ -23,6 +23,7 @@ KERNEL_FUSED = 0
 KERNEL_FQ = 1
 KERNEL_GEMM = 2
 KERNEL_PAIRS = 3
+KERNEL_PAIR_FORCE = 4
 
 _ERRORS = {
     -1: ValueError,

@@ -86,6 +86,9 @@ __device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& 
 }
 
 
+// Smallest per-bin phase step for the plain Chebyshev recurrence in fq_kernel.
+constexpr double kChebMin = 0.05;
+
 // Shared memory of the Debye kernels: the j tile during the pair loop, then
 // reused as the transpose buffer of the end-of-kernel lane reduction.
 constexpr int kRedRow = 65;  // 64 lanes + 1 pad (bank spread)
@@ -345,6 +348,38 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
       const float eps = 4.0f * sh * sh;
       float w0 = sn * g.inv_r;
       float w1 = fmaf(sn, c1, cs * s1) * g.inv_r;
+      // When no kept pair of the wave has dtheta < kChebMin the plain
+      // three-term form w2 = 2 cos(dtheta) w1 - w0 (two ops per bin, rounding
+      // amplified by ~1/sin(dtheta)) is accurate enough; close pairs take the
+      // eps form.  The choice is wave-uniform.
+      if (__all(g.r == 0.0 || dth >= kChebMin)) {
+        const float k2 = 2.0f * c1;
+        if constexpr (WEIGHTED) {
+          const int ej = static_cast<int>(pj.w);
+          const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
+#pragma unroll
+          for (int m4 = 0; m4 < M / 4; ++m4) {
+            const f4v fj = fj4[m4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int m = m4 * 4 + e;
+              acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
+              const float w2 = fmaf(k2, w1, -w0);
+              w0 = w1;
+              w1 = w2;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < M; ++m) {
+            acc[m] += w0;
+            const float w2 = fmaf(k2, w1, -w0);
+            w0 = w1;
+            w1 = w2;
+          }
+        }
+        continue;
+      }
       if constexpr (WEIGHTED) {
         const int ej = static_cast<int>(pj.w);
         const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
@@ -520,17 +555,30 @@ __global__ void uq_kernel(const float* __restrict__ u, int64_t total, int nq, do
   if (e < total) uq[e] = u[e] * static_cast<float>(static_cast<double>(e % nq) * dq);
 }
 
-__global__ void reduce_ranges_kernel(const double* __restrict__ part,
-                                     const Range* __restrict__ ranges, int n_rows, int nq,
-                                     const double* __restrict__ scale, double* __restrict__ out64,
-                                     float* __restrict__ out32) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// out[r][q] = scale[r][q] * sum_{p in ranges[r]} part[p][q] in FP64.  A block
+// covers 64 bins of one row with 16 slices of the partials (p = begin + s,
+// begin + s + 16, ...); the slices are added in a fixed order (deterministic).
+__global__ __launch_bounds__(1024) void reduce_ranges_kernel(const double* __restrict__ part,
+                                                             const Range* __restrict__ ranges,
+                                                             int n_rows, int nq,
+                                                             const double* __restrict__ scale,
+                                                             double* __restrict__ out64,
+                                                             float* __restrict__ out32) {
+  __shared__ double sh[16][65];
+  const int qo = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + qo;
   const int r = blockIdx.y;
-  if (q >= nq || r >= n_rows) return;
   const Range rg = ranges[r];
   double acc = 0.0;
-  for (int p = rg.begin; p < rg.end; ++p) acc += part[(size_t)p * nq + q];
-  const double v = acc * scale[(size_t)r * nq + q];
+  if (q < nq)
+    for (int p = rg.begin + sl; p < rg.end; p += 16) acc += part[(size_t)p * nq + q];
+  sh[sl][qo] = acc;
+  __syncthreads();
+  if (sl != 0 || q >= nq || r >= n_rows) return;
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) t += sh[k][qo];
+  const double v = t * scale[(size_t)r * nq + q];
   if (out64) out64[(size_t)r * nq + q] = v;
   if (out32) out32[(size_t)r * nq + q] = static_cast<float>(v);
 }
@@ -725,7 +773,7 @@ hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range*
                                 int n_rows, int nq, const double* scale, double* out_f64,
                                 float* out_f32) {
   if (n_rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_ranges_kernel, dim3((nq + 255) / 256, n_rows), dim3(256), 0, st, part,
+  hipLaunchKernelGGL(reduce_ranges_kernel, dim3((nq + 63) / 64, n_rows), dim3(1024), 0, st, part,
                      ranges, n_rows, nq, scale, out_f64, out_f32);
   return hipGetLastError();
 }
