@@ -86,8 +86,11 @@ __device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& 
 }
 
 
-// Smallest per-bin phase step for the plain Chebyshev recurrence in fq_kernel.
+// Smallest per-bin phase steps for the plain Chebyshev recurrences: F alone
+// (fq_kernel) and the contracted forces (pair_force_kernel), where
+// S = A - B / r can cancel and magnify the recurrence's rounding.
 constexpr double kChebMin = 0.05;
+constexpr double kChebMinForce = 0.5;
 
 // Shared memory of the Debye kernels: the j tile during the pair loop, then
 // reused as the transpose buffer of the end-of-kernel lane reduction.
@@ -488,9 +491,40 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     const float* tj = tab;
     if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    const bool cheb = __all(pg.r == 0.0 || dth >= kChebMinForce);  // wave-uniform
+    const float k2 = 2.0f * c1;
     for (int c0 = 0; c0 < span; c0 += kPfChunk) {
       float sn, cs;
       sincos_red(static_cast<double>(q_lo + c0) * dth, sn, cs);
+      if (cheb) {
+        // three-term recurrences for sin and cos (one FMA each per bin)
+        float sp = fmaf(sn, c1, -cs * s1), cp = fmaf(cs, c1, sn * s1);
+#pragma unroll
+        for (int m = 0; m < kPfChunk; ++m) {
+          const float2 w = uw[c0 + m];
+          float wu = w.x, wq = w.y;
+          if constexpr (WEIGHTED) {
+            const int q = min(q_lo + c0 + m, nq - 1);
+            const float ff = ti[q] * tj[q];
+            wu *= ff;
+            wq *= ff;
+          }
+          if (m & 1) {
+            a1 = fmaf(wq, cs, a1);
+            b1 = fmaf(wu, sn, b1);
+          } else {
+            a0 = fmaf(wq, cs, a0);
+            b0 = fmaf(wu, sn, b0);
+          }
+          const float sn2 = fmaf(k2, sn, -sp);
+          const float cs2 = fmaf(k2, cs, -cp);
+          sp = sn;
+          cp = cs;
+          sn = sn2;
+          cs = cs2;
+        }
+        continue;
+      }
 #pragma unroll
       for (int m = 0; m < kPfChunk; ++m) {
         const float2 w = uw[c0 + m];
