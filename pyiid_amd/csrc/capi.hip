@@ -385,7 +385,11 @@ int ensure_fused_items(pyiid_ctx* c, int nq) {
                         static_cast<int>(ie)});
     }
   }
+  // ranges[n_ranges]: every item (the F of a single structure)
+  const int n_tile_ranges = static_cast<int>(r.size());
+  r.push_back(Range{0, static_cast<int>(v.size()), 0, static_cast<int>(c->n)});
   CHK(upload_items(c, s, v, r));
+  s.n_ranges = n_tile_ranges;
   s.split = static_cast<int>(split);
   s.key = static_cast<int>(split);
   s.stamp = c->struct_stamp;
@@ -644,13 +648,22 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
   double best = 1e300;
   int best_m = c->grad_m;
   int64_t best_w = c->fused_waves;
+  // a candidate is timed with the reduction of its split partials (the
+  // split changes both)
+  CHK(c->io_b.reserve(sizeof(float) * static_cast<size_t>(c->n) * 3 * std::max(1, t.nq)));
+  auto trial = [&]() {
+    CHK(run_fused(c, slot, qbin, nullptr, false));
+    HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, c->n, t.nq,
+                              t.gs.as<double>(), c->io_b.p, false));
+    return PYIID_OK;
+  };
   for (int m : {32, 16})
     for (int64_t w : {16384LL, 32768LL, 65536LL}) {
       c->grad_m = m;
       c->fused_waves = w;
       double ms = 0.0;
-      CHK(run_fused(c, slot, qbin, nullptr, false));
-      CHK(time_launch(c, [&]() { return run_fused(c, slot, qbin, nullptr, false); }, &ms));
+      CHK(trial());
+      CHK(time_launch(c, trial, &ms));
       if (ms < best) {
         best = ms;
         best_m = m;
@@ -1404,17 +1417,13 @@ int pyiid_grad_fq(pyiid_ctx* c, int slot, double qbin, void* grad, float* fq, in
                             t.gs.as<double>(), gdst, (flags & PYIID_F64) != 0));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, grad, gdst, gbytes, flags));
   if (fq) {
-    // F from all items of the single chain
-    const Range all{0, c->fused_items.n_items, 0, static_cast<int>(n)};
-    CHK(c->io_b.reserve(sizeof(Range)));
-    HIPCHK(hipMemcpyAsync(c->io_b.p, &all, sizeof(Range), hipMemcpyHostToDevice, c->stream));
+    // F from all items of the single chain (the range stored after the tile ranges)
     CHK(c->io_a.reserve(sizeof(float) * nq));
     float* fdst = (flags & PYIID_DEVICE) ? fq : c->io_a.as<float>();
-    HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), c->io_b.as<Range>(), 1, nq,
-                                t.fso.as<double>(), nullptr, fdst));
+    HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(),
+                                c->fused_items.ranges.as<Range>() + c->fused_items.n_ranges, 1,
+                                nq, t.fso.as<double>(), nullptr, fdst));
     if (!(flags & PYIID_DEVICE)) CHK(download(c, fq, fdst, sizeof(float) * nq, flags));
-    // `all` is a host temporary
-    HIPCHK(hipStreamSynchronize(c->stream));
   }
   return finish(c, flags);
 }
