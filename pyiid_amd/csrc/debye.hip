@@ -100,6 +100,14 @@ constexpr int kStoreRow = kGradM + 4;  // 16-byte aligned rows for the store tra
 constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
 constexpr int kSmemBytes = cmax3(kWaves * kRedM * kRedRow * 4, kJTile * 32,
                                  kWaves * 64 * kStoreRow * 4);
+// Fused kernel: per round of kGeoJ j atoms the block's waves (same atoms,
+// different Q chunks) share the per-pair setup through LDS: dtheta (FP64) and
+// 1/r, (q_i - q_j)/r^2, s1/r, s1 r, c1 (FP32), [field][k][lane].
+constexpr int kGeoJ = 16;
+constexpr int kGeoFields = 7;
+constexpr int kGeoBytes = kGeoJ * 64 * (8 + kGeoFields * 4);
+constexpr int kFusedSmemBytes =
+    kSmemBytes > kJTile * 32 + kGeoBytes ? kSmemBytes : kJTile * 32 + kGeoBytes;
 
 // Sum acc[m] over the wave's 64 lanes in FP64 (through LDS); returns, in lane
 // L < M, the total of bin L.  Must be reached by every wave of the block.
@@ -163,8 +171,11 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
     const AtomRec* __restrict__ pos, int n_total, const float* __restrict__ ftab, int n_elem,
     int nq, double dq, const Item* __restrict__ items, float* __restrict__ gpart,
     double* __restrict__ fpart, const float* __restrict__ u, float* __restrict__ force_part) {
-  __shared__ __attribute__((aligned(32))) unsigned char smem[kSmemBytes];
+  __shared__ __attribute__((aligned(32))) unsigned char smem[kFusedSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
+  double* geo_dth = reinterpret_cast<double*>(smem + kJTile * 32);
+  float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32 + kGeoJ * 64 * 8);
+  constexpr int kGS = kGeoJ * 64;  // field stride in geo_f
   __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
   const Item it = items[blockIdx.x];
@@ -209,54 +220,75 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
     if (jt + (int)threadIdx.x < j_end) tile[threadIdx.x] = pos[jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, j_end - jt);
-    if (q0 >= nq) continue;  // wave-uniform
-    for (int jj = 0; jj < cnt; ++jj) {
-      const AtomRec pj = tile[jj];
-      const PairGeom g = pair_geom(pi, pj, lane_ok);
-      const double dth = dq * g.r;
-      float sn, cs, s1, c1;
-      sincos_red(q0d * dth, sn, cs);
-      sincos_red(dth, s1, c1);
-      // Along Q the phasor turns by dtheta (exact rotation, four FMA-class
-      // ops, accurate for every dtheta including close pairs).  It is carried
-      // as (w, cs) with w = sin(Q r) / r, so omega needs no multiply:
-      //   w' = w c1 + cs (s1 / r),  cs' = cs c1 - w (s1 r).
-      const float s1_r = s1 * g.inv_r;
-      const float s1r = s1 * static_cast<float>(g.r);
-      float w = sn * g.inv_r;
-      if constexpr (WEIGHTED) {
-        const int ej = static_cast<int>(pj.w);
-        const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
+    for (int js = 0; js < cnt; js += kGeoJ) {
+      const int kk = min(kGeoJ, cnt - js);
+      // shared setup: wave w forms the pairs of j = js + w, js + w + 4, ...
+      for (int k = wave; k < kk; k += kWaves) {
+        const PairGeom g = pair_geom(pi, tile[js + k], lane_ok);
+        const double dth = dq * g.r;
+        float s1, c1;
+        sincos_red(dth, s1, c1);
+        const int o = k * 64 + lane;
+        geo_dth[o] = dth;
+        geo_f[o] = g.inv_r;
+        geo_f[o + kGS] = g.dpx;
+        geo_f[o + 2 * kGS] = g.dpy;
+        geo_f[o + 3 * kGS] = g.dpz;
+        geo_f[o + 4 * kGS] = s1 * g.inv_r;
+        geo_f[o + 5 * kGS] = s1 * static_cast<float>(g.r);
+        geo_f[o + 6 * kGS] = c1;
+      }
+      __syncthreads();
+      if (q0 < nq) {  // wave-uniform
+        for (int k = 0; k < kk; ++k) {
+          const int o = k * 64 + lane;
+          const double dth = geo_dth[o];
+          const float inv_r = geo_f[o], dpx = geo_f[o + kGS], dpy = geo_f[o + 2 * kGS],
+                      dpz = geo_f[o + 3 * kGS], s1_r = geo_f[o + 4 * kGS],
+                      s1r = geo_f[o + 5 * kGS], c1 = geo_f[o + 6 * kGS];
+          float sn, cs;
+          sincos_red(q0d * dth, sn, cs);
+          // Along Q the phasor turns by dtheta (exact rotation, four FMA-class
+          // ops, accurate for every dtheta including close pairs).  It is
+          // carried as (w, cs) with w = sin(Q r) / r, so omega needs no
+          // multiply:  w' = w c1 + cs (s1 / r),  cs' = cs c1 - w (s1 r).
+          float w = sn * inv_r;
+          if constexpr (WEIGHTED) {
+            const int ej = static_cast<int>(tile[js + k].w);
+            const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
 #pragma unroll
-        for (int m4 = 0; m4 < M / 4; ++m4) {
-          const f4v fj = fj4[m4];
+            for (int m4 = 0; m4 < M / 4; ++m4) {
+              const f4v fj = fj4[m4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int m = m4 * 4 + e;
-            const float ff = fi[m] * fj[e];
-            const float t = fmaf(qv[m], cs, -w) * ff;
-            aF[m] = fmaf(w, ff, aF[m]);
-            ax[m] = fmaf(t, g.dpx, ax[m]);
-            ay[m] = fmaf(t, g.dpy, ay[m]);
-            az[m] = fmaf(t, g.dpz, az[m]);
-            const float w2 = fmaf(w, c1, cs * s1_r);
-            cs = fmaf(cs, c1, -w * s1r);
-            w = w2;
+              for (int e = 0; e < 4; ++e) {
+                const int m = m4 * 4 + e;
+                const float ff = fi[m] * fj[e];
+                const float t = fmaf(qv[m], cs, -w) * ff;
+                aF[m] = fmaf(w, ff, aF[m]);
+                ax[m] = fmaf(t, dpx, ax[m]);
+                ay[m] = fmaf(t, dpy, ay[m]);
+                az[m] = fmaf(t, dpz, az[m]);
+                const float w2 = fmaf(w, c1, cs * s1_r);
+                cs = fmaf(cs, c1, -w * s1r);
+                w = w2;
+              }
+            }
+          } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+              const float t = fmaf(qv[m], cs, -w);
+              aF[m] += w;
+              ax[m] = fmaf(t, dpx, ax[m]);
+              ay[m] = fmaf(t, dpy, ay[m]);
+              az[m] = fmaf(t, dpz, az[m]);
+              const float w2 = fmaf(w, c1, cs * s1_r);
+              cs = fmaf(cs, c1, -w * s1r);
+              w = w2;
+            }
           }
         }
-      } else {
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-          const float t = fmaf(qv[m], cs, -w);
-          aF[m] += w;
-          ax[m] = fmaf(t, g.dpx, ax[m]);
-          ay[m] = fmaf(t, g.dpy, ay[m]);
-          az[m] = fmaf(t, g.dpz, az[m]);
-          const float w2 = fmaf(w, c1, cs * s1_r);
-          cs = fmaf(cs, c1, -w * s1r);
-          w = w2;
-        }
       }
+      __syncthreads();
     }
   }
 
