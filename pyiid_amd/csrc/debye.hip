@@ -108,6 +108,12 @@ constexpr int kGeoFields = 7;
 constexpr int kGeoBytes = kGeoJ * 64 * (8 + kGeoFields * 4);
 constexpr int kFusedSmemBytes =
     kSmemBytes > kJTile * 32 + kGeoBytes ? kSmemBytes : kJTile * 32 + kGeoBytes;
+// F-only kernel: the shared setup is dtheta (FP64) and 1/r, sin, cos, eps of dtheta (FP32).
+constexpr int kFqGeoFields = 4;
+constexpr int kFqGeoBytes = kGeoJ * 64 * (8 + kFqGeoFields * 4);
+constexpr int kFqSmemBytes = kWaves * kRedM * kRedRow * 4 > kJTile * 32 + kFqGeoBytes
+                                 ? kWaves * kRedM * kRedRow * 4
+                                 : kJTile * 32 + kFqGeoBytes;
 
 // Sum acc[m] over the wave's 64 lanes in FP64 (through LDS); returns, in lane
 // L < M, the total of bin L.  Must be reached by every wave of the block.
@@ -334,8 +340,11 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
                                                  const float* __restrict__ ftab, int n_elem,
                                                  int nq, double dq, const Item* __restrict__ items,
                                                  double* __restrict__ fpart) {
-  __shared__ __attribute__((aligned(32))) unsigned char smem[kSmemBytes];
+  __shared__ __attribute__((aligned(32))) unsigned char smem[kFqSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
+  double* geo_dth = reinterpret_cast<double*>(smem + kJTile * 32);
+  float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32 + kGeoJ * 64 * 8);
+  constexpr int kGS = kGeoJ * 64;
   __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
   const Item it = items[blockIdx.x];
@@ -368,77 +377,93 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
     if (jt + (int)threadIdx.x < it.jend) tile[threadIdx.x] = pos[jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, it.jend - jt);
-    if (q0 >= nq) continue;
-    for (int jj = 0; jj < cnt; ++jj) {
-      const AtomRec pj = tile[jj];
-      const PairGeom g = pair_geom(pi, pj, lane_ok && (jt + jj < i));
-      const double dth = dq * g.r;
-      float sn, cs, sh, ch;
-      sincos_red(q0d * dth, sn, cs);
-      sincos_red(0.5 * dth, sh, ch);
-      // cos(dtheta) = 1 - 2 sin^2(dtheta/2) and eps = 2 - 2 cos(dtheta) = 4 sin^2(dtheta/2)
-      // keep their relative accuracy for small dtheta (close pairs).
-      const float s1 = 2.0f * sh * ch;
-      const float c1 = fmaf(-2.0f * sh, sh, 1.0f);
-      const float eps = 4.0f * sh * sh;
-      float w0 = sn * g.inv_r;
-      float w1 = fmaf(sn, c1, cs * s1) * g.inv_r;
-      // When no kept pair of the wave has dtheta < kChebMin the plain
-      // three-term form w2 = 2 cos(dtheta) w1 - w0 (two ops per bin, rounding
-      // amplified by ~1/sin(dtheta)) is accurate enough; close pairs take the
-      // eps form.  The choice is wave-uniform.
-      if (__all(g.r == 0.0 || dth >= kChebMin)) {
-        const float k2 = 2.0f * c1;
-        if constexpr (WEIGHTED) {
-          const int ej = static_cast<int>(pj.w);
-          const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
+    for (int js = 0; js < cnt; js += kGeoJ) {
+      const int kk = min(kGeoJ, cnt - js);
+      // setup shared by the block's waves (same atoms, different Q chunks)
+      for (int k = wave; k < kk; k += kWaves) {
+        const PairGeom g = pair_geom(pi, tile[js + k], lane_ok && (jt + js + k < i));
+        const double dth = dq * g.r;
+        float sh, ch;
+        sincos_red(0.5 * dth, sh, ch);
+        const int o = k * 64 + lane;
+        geo_dth[o] = dth;
+        geo_f[o] = g.inv_r;
+        // cos(dtheta) = 1 - 2 sin^2(dtheta/2) and eps = 2 - 2 cos(dtheta) =
+        // 4 sin^2(dtheta/2) keep their relative accuracy for small dtheta.
+        geo_f[o + kGS] = 2.0f * sh * ch;
+        geo_f[o + 2 * kGS] = fmaf(-2.0f * sh, sh, 1.0f);
+        geo_f[o + 3 * kGS] = 4.0f * sh * sh;
+      }
+      __syncthreads();
+      if (q0 < nq) {  // wave-uniform
+        for (int k = 0; k < kk; ++k) {
+          const int o = k * 64 + lane;
+          const double dth = geo_dth[o];
+          const float inv_r = geo_f[o], s1 = geo_f[o + kGS], c1 = geo_f[o + 2 * kGS],
+                      eps = geo_f[o + 3 * kGS];
+          float sn, cs;
+          sincos_red(q0d * dth, sn, cs);
+          float w0 = sn * inv_r;
+          float w1 = fmaf(sn, c1, cs * s1) * inv_r;
+          // When no kept pair of the wave has dtheta < kChebMin the plain
+          // three-term form w2 = 2 cos(dtheta) w1 - w0 (two ops per bin, rounding
+          // amplified by ~1/sin(dtheta)) is accurate enough; close pairs take the
+          // eps form.  The choice is wave-uniform.
+          if (__all(inv_r == 0.0f || dth >= kChebMin)) {
+            const float k2 = 2.0f * c1;
+            if constexpr (WEIGHTED) {
+              const int ej = static_cast<int>(tile[js + k].w);
+              const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
 #pragma unroll
-          for (int m4 = 0; m4 < M / 4; ++m4) {
-            const f4v fj = fj4[m4];
+              for (int m4 = 0; m4 < M / 4; ++m4) {
+                const f4v fj = fj4[m4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int m = m4 * 4 + e;
-              acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
-              const float w2 = fmaf(k2, w1, -w0);
+                for (int e = 0; e < 4; ++e) {
+                  const int m = m4 * 4 + e;
+                  acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
+                  const float w2 = fmaf(k2, w1, -w0);
+                  w0 = w1;
+                  w1 = w2;
+                }
+              }
+            } else {
+#pragma unroll
+              for (int m = 0; m < M; ++m) {
+                acc[m] += w0;
+                const float w2 = fmaf(k2, w1, -w0);
+                w0 = w1;
+                w1 = w2;
+              }
+            }
+            continue;
+          }
+          if constexpr (WEIGHTED) {
+            const int ej = static_cast<int>(tile[js + k].w);
+            const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
+#pragma unroll
+            for (int m4 = 0; m4 < M / 4; ++m4) {
+              const f4v fj = fj4[m4];
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const int m = m4 * 4 + e;
+                acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
+                const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
+                w0 = w1;
+                w1 = w2;
+              }
+            }
+          } else {
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+              acc[m] += w0;
+              const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
               w0 = w1;
               w1 = w2;
             }
           }
-        } else {
-#pragma unroll
-          for (int m = 0; m < M; ++m) {
-            acc[m] += w0;
-            const float w2 = fmaf(k2, w1, -w0);
-            w0 = w1;
-            w1 = w2;
-          }
-        }
-        continue;
-      }
-      if constexpr (WEIGHTED) {
-        const int ej = static_cast<int>(pj.w);
-        const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
-#pragma unroll
-        for (int m4 = 0; m4 < M / 4; ++m4) {
-          const f4v fj = fj4[m4];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int m = m4 * 4 + e;
-            acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
-            const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
-            w0 = w1;
-            w1 = w2;
-          }
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < M; ++m) {
-          acc[m] += w0;
-          const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
-          w0 = w1;
-          w1 = w2;
         }
       }
+      __syncthreads();
     }
   }
 
