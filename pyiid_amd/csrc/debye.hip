@@ -10,9 +10,13 @@
 //    Each lane accumulates F and the three gradient components for its atom;
 //    no atomics, no K x Q intermediates.
 //  * fq_kernel: unordered pairs (j < i) for F(Q) alone; omega_m = sin(Q_m r)/r
-//    follows the three-term recurrence w_{m+1} = (2 w_m - w_{m-1}) - eps w_m,
-//    eps = 4 sin^2(dtheta/2) from a half-angle sincos (accurate frequency for
-//    close pairs), three VALU ops per pair and Q bin.
+//    follows the three-term recurrence w_{m+1} = 2 cos(dtheta) w_m - w_{m-1}
+//    with 2 cos(dtheta) as a two-float constant (three VALU ops per pair and
+//    Q bin) when every pair of the wave has its step at least 0.2 away from
+//    a multiple of pi, else the Reinsch form
+//    w_{m+1} = w_m + D_{m+1}, D_{m+2} = D_{m+1} - eps w_{m+1}, eps =
+//    4 sin^2(d/2) from a half-angle sincos of d = dtheta - k pi, on the
+//    sequence (-1)^(k m) w_m.
 //  * partial sums leave each wave in FP64 and are reduced across tiles in
 //    FP64 by small deterministic reduction kernels.
 //
@@ -29,6 +33,21 @@ __device__ __forceinline__ float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// sin and cos of |rf| <= pi/4: Taylor series, sin through r^9 and cos
+// through r^10 (truncation < 2e-9).
+__device__ __forceinline__ void sincos_poly(float rf, float& s, float& c) {
+  const float r2 = rf * rf;
+  float ps = fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+  ps = fmaf(r2, ps, 1.0f / 120.0f);
+  ps = fmaf(r2, ps, -1.0f / 6.0f);
+  s = fmaf(rf * r2, ps, rf);
+  float pc = fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+  pc = fmaf(r2, pc, -1.0f / 720.0f);
+  pc = fmaf(r2, pc, 1.0f / 24.0f);
+  pc = fmaf(r2, pc, -0.5f);
+  c = fmaf(r2, pc, 1.0f);
+}
+
 // sin and cos of x (radians, x >= 0, |x| < 1e6).  Cody-Waite reduction by
 // pi/2 in FP64 (two-term split, FMA), then Taylor series on |r| <= pi/4:
 // sin through r^9 and cos through r^10 (truncation < 2e-9).
@@ -39,23 +58,41 @@ __device__ __forceinline__ void sincos_red(double x, float& s, float& c) {
   const double n = rint(x * kTwoOverPi);
   double r = fma(-n, kPio2Hi, x);
   r = fma(-n, kPio2Lo, r);
-  const float rf = static_cast<float>(r);
-  const float r2 = rf * rf;
-  float ps = fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f);
-  ps = fmaf(r2, ps, 1.0f / 120.0f);
-  ps = fmaf(r2, ps, -1.0f / 6.0f);
-  const float sp = fmaf(rf * r2, ps, rf);
-  float pc = fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
-  pc = fmaf(r2, pc, -1.0f / 720.0f);
-  pc = fmaf(r2, pc, 1.0f / 24.0f);
-  pc = fmaf(r2, pc, -0.5f);
-  const float cp = fmaf(r2, pc, 1.0f);
+  float sp, cp;
+  sincos_poly(static_cast<float>(r), sp, cp);
   const int q = static_cast<int>(n) & 3;
   const bool swap = q & 1;
   float so = swap ? cp : sp;
   float co = swap ? sp : cp;
   s = (q & 2) ? -so : so;
   c = ((q + 1) & 2) ? -co : co;
+}
+
+// dtheta = k pi + d with k the nearest integer (|d| <= pi/2): the three-term
+// recurrence y' = 2 cos(dtheta) y - y_prev amplifies rounding by about
+// 1 / |sin(dtheta)| = 1 / sin|d|, so its safe range is set by |d|, near
+// every multiple of pi, not only near 0.
+// sin x for |x| <= pi/4 in FP64 (Taylor series through x^17, truncation
+// below 1e-19).
+__device__ __forceinline__ double sin_poly_f64(double x) {
+  const double x2 = x * x;
+  double p = fma(x2, -1.0 / 355687428096000.0, 1.0 / 1307674368000.0);
+  p = fma(x2, p, -1.0 / 6227020800.0);
+  p = fma(x2, p, 1.0 / 39916800.0);
+  p = fma(x2, p, -1.0 / 362880.0);
+  p = fma(x2, p, 1.0 / 5040.0);
+  p = fma(x2, p, -1.0 / 120.0);
+  p = fma(x2, p, 1.0 / 6.0);
+  return fma(-x * x2, p, x);
+}
+
+__device__ __forceinline__ double pi_reduce(double dth, int& k_odd) {
+  constexpr double kInvPi = 0.31830988618379067154;
+  constexpr double kPiHi = 3.141592653589793116;
+  constexpr double kPiLo = 1.2246467991473532072e-16;
+  const double k = rint(dth * kInvPi);
+  k_odd = static_cast<int>(static_cast<long long>(k) & 1);
+  return fma(-k, kPiLo, fma(-k, kPiHi, dth));
 }
 
 struct PairGeom {
@@ -86,11 +123,14 @@ __device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& 
 }
 
 
-// Smallest per-bin phase steps for the plain Chebyshev recurrences: F alone
-// (fq_kernel) and the contracted forces (pair_force_kernel), where
-// S = A - B / r can cancel and magnify the recurrence's rounding.
-constexpr double kChebMin = 0.05;
+// Smallest distances |d| of the per-bin phase step from a multiple of pi for
+// the plain three-term recurrences: F alone (fq_kernel) and the contracted
+// forces (pair_force_kernel), where S = A - B / r can cancel and magnify the
+// recurrence's rounding.
+constexpr double kChebMin = 0.2;
 constexpr double kChebMinForce = 0.5;
+// the F-only kernel tests eps(d) = 4 sin^2(d / 2) >= 4 sin^2(kChebMin / 2)
+constexpr float kChebEpsMin = 0.039866844f;
 
 // Shared memory of the Debye kernels: the j tile during the pair loop, then
 // reused as the transpose buffer of the end-of-kernel lane reduction.
@@ -108,8 +148,9 @@ constexpr int kGeoFields = 7;
 constexpr int kGeoBytes = kGeoJ * 64 * (8 + kGeoFields * 4);
 constexpr int kFusedSmemBytes =
     kSmemBytes > kJTile * 32 + kGeoBytes ? kSmemBytes : kJTile * 32 + kGeoBytes;
-// F-only kernel: the shared setup is dtheta (FP64) and 1/r, sin, cos, eps of dtheta (FP32).
-constexpr int kFqGeoFields = 4;
+// F-only kernel: the shared setup is dtheta (FP64) and 1/r, sin(dtheta),
+// 2 cos(dtheta) as hi + lo, eps of d and (-1)^k (FP32), dtheta = k pi + d.
+constexpr int kFqGeoFields = 6;
 constexpr int kFqGeoBytes = kGeoJ * 64 * (8 + kFqGeoFields * 4);
 constexpr int kFqSmemBytes = kWaves * kRedM * kRedRow * 4 > kJTile * 32 + kFqGeoBytes
                                  ? kWaves * kRedM * kRedRow * 4
@@ -383,34 +424,49 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
       for (int k = wave; k < kk; k += kWaves) {
         const PairGeom g = pair_geom(pi, tile[js + k], lane_ok && (jt + js + k < i));
         const double dth = dq * g.r;
+        int k_odd;
+        const double d = pi_reduce(dth, k_odd);
         float sh, ch;
-        sincos_red(0.5 * dth, sh, ch);
+        sincos_poly(static_cast<float>(0.5 * d), sh, ch);
+        const float sgn = k_odd ? -1.0f : 1.0f;
         const int o = k * 64 + lane;
         geo_dth[o] = dth;
         geo_f[o] = g.inv_r;
-        // cos(dtheta) = 1 - 2 sin^2(dtheta/2) and eps = 2 - 2 cos(dtheta) =
-        // 4 sin^2(dtheta/2) keep their relative accuracy for small dtheta.
-        geo_f[o + kGS] = 2.0f * sh * ch;
-        geo_f[o + 2 * kGS] = fmaf(-2.0f * sh, sh, 1.0f);
-        geo_f[o + 3 * kGS] = 4.0f * sh * sh;
+        // sin(dtheta) = (-1)^k sin d; 2 cos(dtheta) = (-1)^k (2 - 4 sin^2(d/2))
+        // in FP64, kept as hi + lo so that the three-term recurrence runs at
+        // the exact frequency; eps = 4 sin^2(d/2) keeps its relative accuracy
+        // for small |d|.
+        const double shd = sin_poly_f64(0.5 * d);
+        const double k2 = (k_odd ? -1.0 : 1.0) * fma(-4.0 * shd, shd, 2.0);
+        const float k2hi = static_cast<float>(k2);
+        geo_f[o + kGS] = sgn * 2.0f * sh * ch;
+        geo_f[o + 2 * kGS] = k2hi;
+        geo_f[o + 3 * kGS] = static_cast<float>(k2 - static_cast<double>(k2hi));
+        geo_f[o + 4 * kGS] = 4.0f * sh * sh;
+        geo_f[o + 5 * kGS] = sgn;
       }
       __syncthreads();
       if (q0 < nq) {  // wave-uniform
         for (int k = 0; k < kk; ++k) {
           const int o = k * 64 + lane;
           const double dth = geo_dth[o];
-          const float inv_r = geo_f[o], s1 = geo_f[o + kGS], c1 = geo_f[o + 2 * kGS],
-                      eps = geo_f[o + 3 * kGS];
+          const float inv_r = geo_f[o], s1 = geo_f[o + kGS], k2hi = geo_f[o + 2 * kGS],
+                      k2lo = geo_f[o + 3 * kGS], eps = geo_f[o + 4 * kGS],
+                      sgn = geo_f[o + 5 * kGS];
           float sn, cs;
           sincos_red(q0d * dth, sn, cs);
           float w0 = sn * inv_r;
-          float w1 = fmaf(sn, c1, cs * s1) * inv_r;
-          // When no kept pair of the wave has dtheta < kChebMin the plain
-          // three-term form w2 = 2 cos(dtheta) w1 - w0 (two ops per bin, rounding
-          // amplified by ~1/sin(dtheta)) is accurate enough; close pairs take the
-          // eps form.  The choice is wave-uniform.
-          if (__all(inv_r == 0.0f || dth >= kChebMin)) {
-            const float k2 = 2.0f * c1;
+          // When every kept pair of the wave has |d| >= kChebMin the
+          // three-term form w2 = 2 cos(dtheta) w1 - w0 with the two-float
+          // coefficient (two FMAs per bin, exact frequency, rounding amplified
+          // by at most 1/sin|d| < 5) is accurate enough.
+          // Otherwise the wave runs the Reinsch difference form on the
+          // sequence y_m = (-1)^(k m) w_m = sin(Q0 r + m d) / r:
+          //   y_{m+1} = y_m + D_{m+1},  D_{m+2} = D_{m+1} - eps y_{m+1},
+          // eps = 4 sin^2(d/2), whose rounding stays O(1) in the step count;
+          // odd bins take the sign (-1)^k back.  The choice is wave-uniform.
+          if (__all(inv_r == 0.0f || eps >= kChebEpsMin)) {
+            float w1 = fmaf(sn, 0.5f * k2hi, cs * s1) * inv_r;
             if constexpr (WEIGHTED) {
               const int ej = static_cast<int>(tile[js + k].w);
               const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
@@ -421,7 +477,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
                 for (int e = 0; e < 4; ++e) {
                   const int m = m4 * 4 + e;
                   acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
-                  const float w2 = fmaf(k2, w1, -w0);
+                  const float w2 = fmaf(k2lo, w1, fmaf(k2hi, w1, -w0));
                   w0 = w1;
                   w1 = w2;
                 }
@@ -430,13 +486,20 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
               for (int m = 0; m < M; ++m) {
                 acc[m] += w0;
-                const float w2 = fmaf(k2, w1, -w0);
+                const float w2 = fmaf(k2lo, w1, fmaf(k2hi, w1, -w0));
                 w0 = w1;
                 w1 = w2;
               }
             }
             continue;
           }
+          // D_1 = y_1 - y_0 = (sin(t + d) - sin t) / r
+          //     = 2 sin(d/2) (cos t cos(d/2) - sin t sin(d/2)) / r,
+          // from sin(d/2)^2 = eps / 4 and sin(d/2) cos(d/2) = sgn s1 / 2
+          // (no cancellation for small d).
+          const float sh2 = 0.25f * eps;
+          float dd = fmaf(cs, sgn * s1, -2.0f * sn * sh2) * inv_r;
+          float y = w0;
           if constexpr (WEIGHTED) {
             const int ej = static_cast<int>(tile[js + k].w);
             const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
@@ -446,19 +509,21 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
               for (int e = 0; e < 4; ++e) {
                 const int m = m4 * 4 + e;
-                acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
-                const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
-                w0 = w1;
-                w1 = w2;
+                const float ff = (e & 1) ? fi[m] * fj[e] * sgn : fi[m] * fj[e];
+                acc[m] = fmaf(y, ff, acc[m]);
+                y += dd;
+                dd = fmaf(-eps, y, dd);
               }
             }
           } else {
 #pragma unroll
-            for (int m = 0; m < M; ++m) {
-              acc[m] += w0;
-              const float w2 = fmaf(-eps, w1, fmaf(2.0f, w1, -w0));
-              w0 = w1;
-              w1 = w2;
+            for (int m = 0; m < M; m += 2) {
+              acc[m] += y;
+              y += dd;
+              dd = fmaf(-eps, y, dd);
+              acc[m + 1] = fmaf(sgn, y, acc[m + 1]);
+              y += dd;
+              dd = fmaf(-eps, y, dd);
             }
           }
         }
@@ -548,7 +613,9 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     const float* tj = tab;
     if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-    const bool cheb = __all(pg.r == 0.0 || dth >= kChebMinForce);  // wave-uniform
+    int k_odd;
+    const bool cheb =
+        __all(pg.r == 0.0 || fabs(pi_reduce(dth, k_odd)) >= kChebMinForce);  // wave-uniform
     const float k2 = 2.0f * c1;
     for (int c0 = 0; c0 < span; c0 += kPfChunk) {
       float sn, cs;

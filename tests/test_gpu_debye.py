@@ -107,3 +107,35 @@ def test_tiny_structures(ctx, n):
         Q = oracle.q_values(qbin, nq)
         assert rel(fq, np.sin(Q * r) / r) < TOL  # dimer closed form
         assert rel(g, oracle.grad_fq(pos, per_atom, qbin)) < TOL
+
+
+@pytest.mark.parametrize("k,off", [(1, 5e-4), (1, -3e-3), (1, 0.02), (1, 0.3), (2, 1e-3),
+                                   (3, -0.01)])
+def test_steps_near_multiples_of_pi(ctx, k, off):
+    """A square of side s with qbin * s near k pi, plus an atom off its plane.
+    The three-term recurrence amplifies rounding by ~min(m, 1 / |sin(qbin r)|)
+    after m bins, near every multiple of pi as near 0, so such pairs take the
+    accurate forms (F-only kernel: the Reinsch difference form; contracted
+    forces: the rotation).  F, the gradient and the F(Q)-Rw forces (target:
+    the structure shrunk by 0.1 %) against the oracle; the forces of this
+    configuration move by ~1e-6 for 1e-6 noise in F or its gradient."""
+    qbin, nq, q_first = 0.1, 250, 10
+    side = (k * np.pi + off) / qbin
+    pos = np.array([[0, 0, 0], [side, 0, 0], [0, side, 0], [side, side, 0],
+                    [0.37 * side, 0.61 * side, 2.5]], np.float32)
+    n = len(pos)
+    per_atom = load(ctx, pos, np.full(n, 79), qbin, nq)
+    f_want = oracle.fq(pos, per_atom, qbin)
+    fq = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    g_want = oracle.grad_fq(pos, per_atom, qbin)
+    fq_fused = np.empty(nq, np.float32)
+    g = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, f64=True, fq_out=fq_fused)
+    target = oracle.fq((pos * np.float32(0.999)).astype(np.float32), per_atom, qbin)[q_first:]
+    e, f = ctx.fq_energy_forces(_lib.SLOT_FQ, qbin, q_first, target, _lib.POT_RW, 1.0, n)
+    f_want_forces = oracle.wrap_grad_rw(g_want[:, :, q_first:], f_want[q_first:], target)
+    print("k=%d off=%g: F %.2e F(fused) %.2e grad %.2e forces %.2e" % (
+        k, off, rel(fq, f_want), rel(fq_fused, f_want), rel(g, g_want), rel(f, f_want_forces)))
+    assert rel(fq, f_want) < TOL and rel(fq_fused, f_want) < TOL
+    assert rel(g, g_want) < TOL
+    assert abs(e - oracle.get_rw(target, f_want[q_first:])[0]) < 1e-6
+    assert rel(f, f_want_forces) < TOL
