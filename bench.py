@@ -9,6 +9,7 @@ to device memory.  Prints one JSON line.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -165,7 +166,7 @@ def kernel_suite(reps=3):
     """Each hot kernel on its own workload, one warm-up and `reps` timed
     launches (HIP events on the context stream), for profiling runs:
     F-only and pair stage on the 6,699-atom sphere (1,250 bins; all 22.4 M
-    pairs), the transform GEMM on its grad-G rows (3 N x 330 -> 4,000 r
+    pairs), F-only and fused F + gradient on a 20,000-atom sphere, the transform GEMM on its grad-G rows (3 N x 330 -> 4,000 r
     points), contracted forces and learned-integrator forces on 64 chains of
     5,000 atoms."""
     import torch
@@ -196,6 +197,30 @@ def kernel_suite(reps=3):
     ms = timed(ctx, _lib.KERNEL_FQ, lambda: ctx.fq(_lib.SLOT_FQ, QBIN, NQ, out=fq.data_ptr(),
                                                    flags=dev))
     out["fq_kernel"] = {"ms": ms, "pair_q_per_s": pairs * NQ / (ms * 1e-3)}
+    # both Debye kernels on a 20k-atom particle (Au FCC sphere r = 43.3 A)
+    from pyiid_amd import structures
+    from pyiid_amd.formfactors import element_table
+    pos20 = structures.jitter(structures.fcc_sphere(43.3, LATTICE), SIGMA, SEED).astype(np.float32)
+    t20, i20, _ = element_table(np.full(len(pos20), 79), QBIN, NQ)
+    c20 = _lib.Context(0)
+    c20.set_positions(pos20)
+    c20.set_elements(i20)
+    c20.set_scatter(_lib.SLOT_FQ, t20)
+    n20 = len(pos20)
+    p20 = n20 * (n20 - 1) // 2
+    g20 = torch.empty((n20, 3, NQ), dtype=torch.float32, device="cuda")
+    f20 = torch.empty(NQ, dtype=torch.float32, device="cuda")
+    ms = timed(c20, _lib.KERNEL_FQ, lambda: c20.fq(_lib.SLOT_FQ, QBIN, NQ, out=f20.data_ptr(),
+                                                  flags=dev))
+    out["fq_kernel_20k"] = {"ms": ms, "atoms": n20, "pair_q_per_s": p20 * NQ / (ms * 1e-3)}
+    ms = timed(c20, _lib.KERNEL_FUSED, lambda: _lib.check(c20.lib.pyiid_grad_fq(
+        c20.handle, _lib.SLOT_FQ, ctypes.c_double(QBIN), ctypes.c_void_p(g20.data_ptr()),
+        ctypes.c_void_p(f20.data_ptr()), dev)))
+    out["fused_grad_kernel_20k"] = {"ms": ms, "atoms": n20,
+                                    "pair_q_per_s": p20 * NQ / (ms * 1e-3),
+                                    "tflops_alg": FLOP_PER_PAIR_Q * p20 * NQ / (ms * 1e-3) / 1e12}
+    del g20, f20
+    c20.close()
     d = torch.empty((pairs, 3), dtype=torch.float32, device="cuda")
     r = torch.empty(pairs, dtype=torch.float32, device="cuda")
     ms = timed(ctx, _lib.KERNEL_PAIRS, lambda: _lib.check(ctx.lib.pyiid_pair_geometry(

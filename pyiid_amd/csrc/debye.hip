@@ -1,19 +1,22 @@
 // Debye scattering kernels for gfx950 (CDNA4, wave64).
 //
 //  * fused_grad_kernel: ordered pairs (i, j), one lane per atom i, a wave
-//    covers 64 atoms x kGradM consecutive Q bins; j atoms are staged in LDS
-//    and read as wave-wide broadcasts.  Per (i, j) the geometry, the phase
+//    covers 64 atoms x m consecutive Q bins; j atoms are staged in LDS and
+//    read as wave-wide broadcasts, and the per-pair setup is shared by the
+//    block's waves through LDS.  Per (i, j) the geometry, the phase
 //    theta0 = Q0 r and the step dtheta = dq r are formed in FP64 and reduced
 //    with a Cody-Waite split of pi/2; sin/cos come from Taylor polynomials in
-//    FP32.  Along Q the phasor (cos, sin) is advanced by the exact rotation
-//    by dtheta (angle-addition recurrence), so the inner loop is FMA only.
+//    FP32.  Along Q, sin(Q r)/r and cos(Q r) follow the three-term
+//    recurrence with 2 cos(dtheta) as a two-float constant (FP64 setup, the
+//    low part applied every fourth bin), so the inner loop is FMA only.
 //    Each lane accumulates F and the three gradient components for its atom;
 //    no atomics, no K x Q intermediates.
 //  * fq_kernel: unordered pairs (j < i) for F(Q) alone; omega_m = sin(Q_m r)/r
 //    follows the three-term recurrence w_{m+1} = 2 cos(dtheta) w_m - w_{m-1}
-//    with 2 cos(dtheta) as a two-float constant (three VALU ops per pair and
-//    Q bin) when every pair of the wave has its step at least 0.2 away from
-//    a multiple of pi, else the Reinsch form
+//    with 2 cos(dtheta) as a two-float constant, the low part applied every
+//    fourth bin (2.25 VALU ops per pair and Q bin), when every pair of the
+//    wave has its step at least 0.2 away from a multiple of pi, else the
+//    Reinsch form
 //    w_{m+1} = w_m + D_{m+1}, D_{m+2} = D_{m+1} - eps w_{m+1}, eps =
 //    4 sin^2(d/2) from a half-angle sincos of d = dtheta - k pi, on the
 //    sequence (-1)^(k m) w_m.
@@ -95,6 +98,19 @@ __device__ __forceinline__ double pi_reduce(double dth, int& k_odd) {
   return fma(-k, kPiLo, fma(-k, kPiHi, dth));
 }
 
+// 2 cos(dtheta) as hi + lo (FP32 each, from FP64): the three-term
+// recurrence y' = (hi + lo) y - y_prev then runs at the exact frequency.
+// With hi alone the frequency error ~ulp / |sin(dtheta)| becomes a phase
+// drift that grows with the step count.
+__device__ __forceinline__ void two_cos(double dth, float& hi, float& lo) {
+  int k_odd;
+  const double d = pi_reduce(dth, k_odd);
+  const double shd = sin_poly_f64(0.5 * d);
+  const double k2 = (k_odd ? -1.0 : 1.0) * fma(-4.0 * shd, shd, 2.0);
+  hi = static_cast<float>(k2);
+  lo = static_cast<float>(k2 - static_cast<double>(hi));
+}
+
 struct PairGeom {
   float inv_r;   // 1/r (0 for excluded pairs)
   float dpx, dpy, dpz;  // (q_i - q_j) / r^2
@@ -123,13 +139,11 @@ __device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& 
 }
 
 
-// Smallest distances |d| of the per-bin phase step from a multiple of pi for
-// the plain three-term recurrences: F alone (fq_kernel) and the contracted
-// forces (pair_force_kernel), where S = A - B / r can cancel and magnify the
-// recurrence's rounding.
+// Smallest distance |d| of the per-bin phase step from a multiple of pi for
+// the three-term recurrence of the F-only kernel (64 bins per anchor); closer
+// steps take the Reinsch form.  The kernel tests
+// eps(d) = 4 sin^2(d / 2) >= 4 sin^2(kChebMin / 2).
 constexpr double kChebMin = 0.2;
-constexpr double kChebMinForce = 0.5;
-// the F-only kernel tests eps(d) = 4 sin^2(d / 2) >= 4 sin^2(kChebMin / 2)
 constexpr float kChebEpsMin = 0.039866844f;
 
 // Shared memory of the Debye kernels: the j tile during the pair loop, then
@@ -273,17 +287,18 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
       for (int k = wave; k < kk; k += kWaves) {
         const PairGeom g = pair_geom(pi, tile[js + k], lane_ok);
         const double dth = dq * g.r;
-        float s1, c1;
+        float s1, c1, k2hi, k2lo;
         sincos_red(dth, s1, c1);
+        two_cos(dth, k2hi, k2lo);
         const int o = k * 64 + lane;
         geo_dth[o] = dth;
         geo_f[o] = g.inv_r;
         geo_f[o + kGS] = g.dpx;
         geo_f[o + 2 * kGS] = g.dpy;
         geo_f[o + 3 * kGS] = g.dpz;
-        geo_f[o + 4 * kGS] = s1 * g.inv_r;
-        geo_f[o + 5 * kGS] = s1 * static_cast<float>(g.r);
-        geo_f[o + 6 * kGS] = c1;
+        geo_f[o + 4 * kGS] = s1;
+        geo_f[o + 5 * kGS] = k2hi;
+        geo_f[o + 6 * kGS] = 4.0f * k2lo;
       }
       __syncthreads();
       if (q0 < nq) {  // wave-uniform
@@ -291,15 +306,20 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
           const int o = k * 64 + lane;
           const double dth = geo_dth[o];
           const float inv_r = geo_f[o], dpx = geo_f[o + kGS], dpy = geo_f[o + 2 * kGS],
-                      dpz = geo_f[o + 3 * kGS], s1_r = geo_f[o + 4 * kGS],
-                      s1r = geo_f[o + 5 * kGS], c1 = geo_f[o + 6 * kGS];
+                      dpz = geo_f[o + 3 * kGS], s1 = geo_f[o + 4 * kGS],
+                      k2 = geo_f[o + 5 * kGS], lo4 = geo_f[o + 6 * kGS];
           float sn, cs;
           sincos_red(q0d * dth, sn, cs);
-          // Along Q the phasor turns by dtheta (exact rotation, four FMA-class
-          // ops, accurate for every dtheta including close pairs).  It is
-          // carried as (w, cs) with w = sin(Q r) / r, so omega needs no
-          // multiply:  w' = w c1 + cs (s1 / r),  cs' = cs c1 - w (s1 r).
+          // Along Q, w = sin(Q r) / r and cs = cos(Q r) follow the three-term
+          // recurrence y' = 2 cos(dtheta) y - y_prev (one FMA each per bin)
+          // with 2 cos(dtheta) = k2 + lo: the low part is added every fourth
+          // bin as 4 lo, which keeps the frequency exact on average; the
+          // remaining error is bounded (about 50 ulp after 32 bins for close
+          // pairs) instead of the drift of an FP32-rounded coefficient.
+          const float c1 = 0.5f * k2;
           float w = sn * inv_r;
+          float wp = fmaf(sn, c1, -cs * s1) * inv_r;  // sin(Q0 r - dtheta) / r
+          float cp = fmaf(cs, c1, sn * s1);           // cos(Q0 r - dtheta)
           if constexpr (WEIGHTED) {
             const int ej = static_cast<int>(tile[js + k].w);
             const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
@@ -315,9 +335,16 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
                 ax[m] = fmaf(t, dpx, ax[m]);
                 ay[m] = fmaf(t, dpy, ay[m]);
                 az[m] = fmaf(t, dpz, az[m]);
-                const float w2 = fmaf(w, c1, cs * s1_r);
-                cs = fmaf(cs, c1, -w * s1r);
-                w = w2;
+                float wn = fmaf(k2, w, -wp);
+                float cn = fmaf(k2, cs, -cp);
+                if (e == 3) {
+                  wn = fmaf(lo4, w, wn);
+                  cn = fmaf(lo4, cs, cn);
+                }
+                wp = w;
+                cp = cs;
+                w = wn;
+                cs = cn;
               }
             }
           } else {
@@ -328,9 +355,16 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
               ax[m] = fmaf(t, dpx, ax[m]);
               ay[m] = fmaf(t, dpy, ay[m]);
               az[m] = fmaf(t, dpz, az[m]);
-              const float w2 = fmaf(w, c1, cs * s1_r);
-              cs = fmaf(cs, c1, -w * s1r);
-              w = w2;
+              float wn = fmaf(k2, w, -wp);
+              float cn = fmaf(k2, cs, -cp);
+              if ((m & 3) == 3) {
+                wn = fmaf(lo4, w, wn);
+                cn = fmaf(lo4, cs, cn);
+              }
+              wp = w;
+              cp = cs;
+              w = wn;
+              cs = cn;
             }
           }
         }
@@ -441,7 +475,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
         const float k2hi = static_cast<float>(k2);
         geo_f[o + kGS] = sgn * 2.0f * sh * ch;
         geo_f[o + 2 * kGS] = k2hi;
-        geo_f[o + 3 * kGS] = static_cast<float>(k2 - static_cast<double>(k2hi));
+        geo_f[o + 3 * kGS] = 4.0f * static_cast<float>(k2 - static_cast<double>(k2hi));
         geo_f[o + 4 * kGS] = 4.0f * sh * sh;
         geo_f[o + 5 * kGS] = sgn;
       }
@@ -451,15 +485,16 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
           const int o = k * 64 + lane;
           const double dth = geo_dth[o];
           const float inv_r = geo_f[o], s1 = geo_f[o + kGS], k2hi = geo_f[o + 2 * kGS],
-                      k2lo = geo_f[o + 3 * kGS], eps = geo_f[o + 4 * kGS],
+                      lo4 = geo_f[o + 3 * kGS], eps = geo_f[o + 4 * kGS],
                       sgn = geo_f[o + 5 * kGS];
           float sn, cs;
           sincos_red(q0d * dth, sn, cs);
           float w0 = sn * inv_r;
           // When every kept pair of the wave has |d| >= kChebMin the
           // three-term form w2 = 2 cos(dtheta) w1 - w0 with the two-float
-          // coefficient (two FMAs per bin, exact frequency, rounding amplified
-          // by at most 1/sin|d| < 5) is accurate enough.
+          // coefficient (one FMA per bin, the low part added every fourth bin:
+          // exact frequency on average, rounding amplified by at most
+          // 1/sin|d| < 5) is accurate enough.
           // Otherwise the wave runs the Reinsch difference form on the
           // sequence y_m = (-1)^(k m) w_m = sin(Q0 r + m d) / r:
           //   y_{m+1} = y_m + D_{m+1},  D_{m+2} = D_{m+1} - eps y_{m+1},
@@ -477,7 +512,8 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
                 for (int e = 0; e < 4; ++e) {
                   const int m = m4 * 4 + e;
                   acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
-                  const float w2 = fmaf(k2lo, w1, fmaf(k2hi, w1, -w0));
+                  float w2 = fmaf(k2hi, w1, -w0);
+                  if (e == 3) w2 = fmaf(lo4, w1, w2);
                   w0 = w1;
                   w1 = w2;
                 }
@@ -486,7 +522,8 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
               for (int m = 0; m < M; ++m) {
                 acc[m] += w0;
-                const float w2 = fmaf(k2lo, w1, fmaf(k2hi, w1, -w0));
+                float w2 = fmaf(k2hi, w1, -w0);
+                if ((m & 3) == 3) w2 = fmaf(lo4, w1, w2);
                 w0 = w1;
                 w1 = w2;
               }
@@ -608,47 +645,21 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     const bool keep = i_ok && j_ok && !(diag && s == 32 && lane >= 32);
     const PairGeom pg = pair_geom(pi, pj, keep);
     const double dth = dq * pg.r;
-    float s1, c1;
+    float s1, c1, k2, lo;
     sincos_red(dth, s1, c1);
+    two_cos(dth, k2, lo);
+    const float lo4 = 4.0f * lo;
     const float* tj = tab;
     if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-    int k_odd;
-    const bool cheb =
-        __all(pg.r == 0.0 || fabs(pi_reduce(dth, k_odd)) >= kChebMinForce);  // wave-uniform
-    const float k2 = 2.0f * c1;
     for (int c0 = 0; c0 < span; c0 += kPfChunk) {
       float sn, cs;
       sincos_red(static_cast<double>(q_lo + c0) * dth, sn, cs);
-      if (cheb) {
-        // three-term recurrences for sin and cos (one FMA each per bin)
-        float sp = fmaf(sn, c1, -cs * s1), cp = fmaf(cs, c1, sn * s1);
-#pragma unroll
-        for (int m = 0; m < kPfChunk; ++m) {
-          const float2 w = uw[c0 + m];
-          float wu = w.x, wq = w.y;
-          if constexpr (WEIGHTED) {
-            const int q = min(q_lo + c0 + m, nq - 1);
-            const float ff = ti[q] * tj[q];
-            wu *= ff;
-            wq *= ff;
-          }
-          if (m & 1) {
-            a1 = fmaf(wq, cs, a1);
-            b1 = fmaf(wu, sn, b1);
-          } else {
-            a0 = fmaf(wq, cs, a0);
-            b0 = fmaf(wu, sn, b0);
-          }
-          const float sn2 = fmaf(k2, sn, -sp);
-          const float cs2 = fmaf(k2, cs, -cp);
-          sp = sn;
-          cp = cs;
-          sn = sn2;
-          cs = cs2;
-        }
-        continue;
-      }
+      // sin and cos follow the three-term recurrence (one FMA each per bin)
+      // with 2 cos(dtheta) = k2 + lo, the low part added every fourth bin:
+      // exact frequency on average, bounded error for every dtheta (close
+      // pairs and steps near multiples of pi included), no per-wave branch.
+      float sp = fmaf(sn, c1, -cs * s1), cp = fmaf(cs, c1, sn * s1);
 #pragma unroll
       for (int m = 0; m < kPfChunk; ++m) {
         const float2 w = uw[c0 + m];
@@ -666,9 +677,16 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
           a0 = fmaf(wq, cs, a0);
           b0 = fmaf(wu, sn, b0);
         }
-        const float sn2 = fmaf(sn, c1, cs * s1);
-        cs = fmaf(cs, c1, -sn * s1);
+        float sn2 = fmaf(k2, sn, -sp);
+        float cs2 = fmaf(k2, cs, -cp);
+        if ((m & 3) == 3) {
+          sn2 = fmaf(lo4, sn, sn2);
+          cs2 = fmaf(lo4, cs, cs2);
+        }
+        sp = sn;
+        cp = cs;
         sn = sn2;
+        cs = cs2;
       }
     }
     const float S = (a0 + a1) - (b0 + b1) * pg.inv_r;
