@@ -102,13 +102,17 @@ __device__ __forceinline__ double pi_reduce(double dth, int& k_odd) {
 // recurrence y' = (hi + lo) y - y_prev then runs at the exact frequency.
 // With hi alone the frequency error ~ulp / |sin(dtheta)| becomes a phase
 // drift that grows with the step count.
-__device__ __forceinline__ void two_cos(double dth, float& hi, float& lo) {
+// Also sin(dtheta) in FP32 from the same reduction: (-1)^k 2 sin(d/2) cos(d/2).
+__device__ __forceinline__ void two_cos(double dth, float& hi, float& lo, float& s1) {
   int k_odd;
   const double d = pi_reduce(dth, k_odd);
   const double shd = sin_poly_f64(0.5 * d);
   const double k2 = (k_odd ? -1.0 : 1.0) * fma(-4.0 * shd, shd, 2.0);
   hi = static_cast<float>(k2);
   lo = static_cast<float>(k2 - static_cast<double>(hi));
+  const float shf = static_cast<float>(shd);
+  const float s = 2.0f * shf * __builtin_amdgcn_sqrtf(fmaf(-shf, shf, 1.0f));
+  s1 = k_odd ? -s : s;
 }
 
 struct PairGeom {
@@ -155,11 +159,12 @@ constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b >
 constexpr int kSmemBytes = cmax3(kWaves * kRedM * kRedRow * 4, kJTile * 32,
                                  kWaves * 64 * kStoreRow * 4);
 // Fused kernel: per round of kGeoJ j atoms the block's waves (same atoms,
-// different Q chunks) share the per-pair setup through LDS: dtheta (FP64) and
-// 1/r, (q_i - q_j)/r^2, s1/r, s1 r, c1 (FP32), [field][k][lane].
+// different Q chunks) share the per-pair setup through LDS (FP32,
+// [field][k][lane]): 1/r, (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi and
+// 4 lo, and the anchor sin, cos(Q0 r) of every wave's Q chunk.
 constexpr int kGeoJ = 16;
-constexpr int kGeoFields = 7;
-constexpr int kGeoBytes = kGeoJ * 64 * (8 + kGeoFields * 4);
+constexpr int kGeoFields = 7 + 2 * kWaves;
+constexpr int kGeoBytes = kGeoJ * 64 * kGeoFields * 4;
 constexpr int kFusedSmemBytes =
     kSmemBytes > kJTile * 32 + kGeoBytes ? kSmemBytes : kJTile * 32 + kGeoBytes;
 // F-only kernel: the shared setup is dtheta (FP64) and 1/r, sin(dtheta),
@@ -234,8 +239,7 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
     double* __restrict__ fpart, const float* __restrict__ u, float* __restrict__ force_part) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFusedSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
-  double* geo_dth = reinterpret_cast<double*>(smem + kJTile * 32);
-  float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32 + kGeoJ * 64 * 8);
+  float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
   constexpr int kGS = kGeoJ * 64;  // field stride in geo_f
   __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
@@ -275,7 +279,8 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
 #pragma unroll
   for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = 0.0f;
 
-  const double q0d = static_cast<double>(q0);
+  // first bin of the block (its waves take consecutive chunks of M bins)
+  const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
     if (jt + (int)threadIdx.x < j_end) tile[threadIdx.x] = pos[jt + threadIdx.x];
@@ -287,11 +292,9 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
       for (int k = wave; k < kk; k += kWaves) {
         const PairGeom g = pair_geom(pi, tile[js + k], lane_ok);
         const double dth = dq * g.r;
-        float s1, c1, k2hi, k2lo;
-        sincos_red(dth, s1, c1);
-        two_cos(dth, k2hi, k2lo);
+        float s1, k2hi, k2lo;
+        two_cos(dth, k2hi, k2lo, s1);
         const int o = k * 64 + lane;
-        geo_dth[o] = dth;
         geo_f[o] = g.inv_r;
         geo_f[o + kGS] = g.dpx;
         geo_f[o + 2 * kGS] = g.dpy;
@@ -299,17 +302,28 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
         geo_f[o + 4 * kGS] = s1;
         geo_f[o + 5 * kGS] = k2hi;
         geo_f[o + 6 * kGS] = 4.0f * k2lo;
+        // the anchors sin, cos(Q0 r) of the block's kWaves Q chunks: the
+        // first reduced in FP64, the others by FP32 rotations by M dtheta
+        float as, ac, rs, rc;
+        sincos_red(qbase * dth, as, ac);
+        sincos_red(static_cast<double>(M) * dth, rs, rc);
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          geo_f[o + (7 + 2 * w) * kGS] = as;
+          geo_f[o + (8 + 2 * w) * kGS] = ac;
+          const float as2 = fmaf(as, rc, ac * rs);
+          ac = fmaf(ac, rc, -as * rs);
+          as = as2;
+        }
       }
       __syncthreads();
       if (q0 < nq) {  // wave-uniform
         for (int k = 0; k < kk; ++k) {
           const int o = k * 64 + lane;
-          const double dth = geo_dth[o];
           const float inv_r = geo_f[o], dpx = geo_f[o + kGS], dpy = geo_f[o + 2 * kGS],
                       dpz = geo_f[o + 3 * kGS], s1 = geo_f[o + 4 * kGS],
                       k2 = geo_f[o + 5 * kGS], lo4 = geo_f[o + 6 * kGS];
-          float sn, cs;
-          sincos_red(q0d * dth, sn, cs);
+          float sn = geo_f[o + (7 + 2 * wave) * kGS], cs = geo_f[o + (8 + 2 * wave) * kGS];
           // Along Q, w = sin(Q r) / r and cs = cos(Q r) follow the three-term
           // recurrence y' = 2 cos(dtheta) y - y_prev (one FMA each per bin)
           // with 2 cos(dtheta) = k2 + lo: the low part is added every fourth
@@ -645,10 +659,9 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     const bool keep = i_ok && j_ok && !(diag && s == 32 && lane >= 32);
     const PairGeom pg = pair_geom(pi, pj, keep);
     const double dth = dq * pg.r;
-    float s1, c1, k2, lo;
-    sincos_red(dth, s1, c1);
-    two_cos(dth, k2, lo);
-    const float lo4 = 4.0f * lo;
+    float s1, k2, lo;
+    two_cos(dth, k2, lo, s1);
+    const float c1 = 0.5f * k2, lo4 = 4.0f * lo;
     const float* tj = tab;
     if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
