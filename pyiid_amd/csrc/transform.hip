@@ -84,6 +84,112 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const void* __restrict__ a
       }
 }
 
+// The same product for tall A (the gradient rows): block tile 128 x 128,
+// 4 waves of 64 x 64 (4 x 4 MFMA 16x16x4 tiles), K in chunks of 16 through
+// double-buffered LDS; the next chunk's global loads are in flight during the
+// current chunk's MFMAs, one barrier per chunk.  Blocks are dispatched round
+// robin to the 8 XCDs (block id mod 8); the block -> tile map gives each XCD
+// its own column tiles, so the rows of B (the transform) that it reads stay
+// in that XCD's L2 while it sweeps the row tiles.
+constexpr int kG2 = 128, kG2K = 16, kG2Ld = kG2K + 1;
+
+template <bool A64>
+__global__ __launch_bounds__(256) void gemm_nt128_kernel(const void* __restrict__ a_raw,
+                                                         int64_t m, int k,
+                                                         const float* __restrict__ b, int n,
+                                                         int n_ct, int64_t n_rt,
+                                                         double* __restrict__ c64,
+                                                         float* __restrict__ c32) {
+  __shared__ float as[2][kG2 * kG2Ld];
+  __shared__ float bs[2][kG2 * kG2Ld];
+  const int64_t bid = blockIdx.x;
+  const int ct_per = (n_ct + 7) / 8;
+  const int ct = static_cast<int>(bid % 8) + 8 * static_cast<int>((bid / 8) % ct_per);
+  const int64_t rt = (bid / 8) / ct_per;
+  if (ct >= n_ct || rt >= n_rt) return;  // padding block (uniform, before any barrier)
+  const int64_t bm = rt * kG2;
+  const int bn = ct * kG2;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  float ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = threadIdx.x + e * 256;
+      const int row = idx / kG2K, kk = idx % kG2K;
+      const int64_t ar = bm + row;
+      float av = 0.0f;
+      if (ar < m && k0 + kk < k) {
+        if constexpr (A64)
+          av = static_cast<float>(static_cast<const double*>(a_raw)[ar * k + k0 + kk]);
+        else
+          av = static_cast<const float*>(a_raw)[ar * k + k0 + kk];
+      }
+      ra[e] = av;
+      const int br = bn + row;
+      rb[e] = (br < n && k0 + kk < k) ? b[static_cast<int64_t>(br) * k + k0 + kk] : 0.0f;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = threadIdx.x + e * 256;
+      const int row = idx / kG2K, kk = idx % kG2K;
+      as[buf][row * kG2Ld + kk] = ra[e];
+      bs[buf][row * kG2Ld + kk] = rb[e];
+    }
+  };
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  const int nk = (k + kG2K - 1) / kG2K;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int c = 0; c < nk; ++c) {
+    const int cur = c & 1;
+    if (c + 1 < nk) load((c + 1) * kG2K);
+    const float* a_s = as[cur];
+    const float* b_s = bs[cur];
+#pragma unroll
+    for (int kk = 0; kk < kG2K; kk += 4) {
+      float af[4], bf[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        af[x] = a_s[(wm * 64 + x * 16 + (lane & 15)) * kG2Ld + kk + (lane >> 4)];
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+        bf[y] = b_s[(wn * 64 + y * 16 + (lane & 15)) * kG2Ld + kk + (lane >> 4)];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[x], bf[y], acc[x][y], 0, 0, 0);
+    }
+    if (c + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = bm + wm * 64 + x * 16 + 4 * (lane >> 4) + v;
+        const int col = bn + wn * 64 + y * 16 + (lane & 15);
+        if (row < m && col < n) {
+          if (c64) c64[row * n + col] = static_cast<double>(acc[x][y][v]);
+          if (c32) c32[row * n + col] = acc[x][y][v];
+        }
+      }
+}
+
 __device__ double block_sum(double v, double* sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -211,6 +317,20 @@ int blocks_for(int64_t total, int block) {
 
 hipError_t launch_gemm_nt(hipStream_t st, const void* a, bool a_f64, int64_t m, int k,
                           const float* b, int n, double* c_f64, float* c_f32) {
+  if (m >= 1024) {
+    const int n_ct = (n + kG2 - 1) / kG2;
+    const int64_t n_rt = (m + kG2 - 1) / kG2;
+    const int64_t blocks = 8LL * ((n_ct + 7) / 8) * n_rt;
+    if (blocks > 2147483647LL) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(blocks));
+    if (a_f64)
+      hipLaunchKernelGGL(gemm_nt128_kernel<true>, grid, dim3(256), 0, st, a, m, k, b, n, n_ct,
+                         n_rt, c_f64, c_f32);
+    else
+      hipLaunchKernelGGL(gemm_nt128_kernel<false>, grid, dim3(256), 0, st, a, m, k, b, n, n_ct,
+                         n_rt, c_f64, c_f32);
+    return hipGetLastError();
+  }
   const int64_t gy = (m + 63) / 64;
   if (gy > 2147483647LL) return hipErrorInvalidValue;
   dim3 grid((n + 63) / 64, static_cast<unsigned>(gy));

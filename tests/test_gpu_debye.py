@@ -139,3 +139,25 @@ def test_steps_near_multiples_of_pi(ctx, k, off):
     assert rel(g, g_want) < TOL
     assert abs(e - oracle.get_rw(target, f_want[q_first:])[0]) < 1e-6
     assert rel(f, f_want_forces) < TOL
+
+
+@pytest.mark.parametrize("qbin,nq", [(0.03, 834), (0.003, 8334)])
+def test_contracted_forces_both_kernels(ctx, qbin, nq):
+    """F(Q)-Rw energy and forces of a Pt/Ni alloy.  At 834 bins the element
+    table fits the unordered-pair force kernel's LDS; at 8,334 bins
+    (2 x 8,334 floats > 64 KB) the call takes the fused kernel's contracted
+    variant instead.  Both against the oracle."""
+    base = structures.fcc_nearest(43, a=3.724)
+    pos = structures.jitter(base, 0.05, seed=4).astype(np.float32)
+    numbers = structures.random_alloy(len(pos), 78, 28, seed=6)
+    n, q_first = len(pos), 20
+    per_atom = load(ctx, pos, numbers, qbin, nq)
+    target = oracle.fq((base * np.float32(1.002)).astype(np.float32), per_atom, qbin)[q_first:]
+    e, f = ctx.fq_energy_forces(_lib.SLOT_FQ, qbin, q_first, target, _lib.POT_RW, 10.0, n)
+    f_want_fq = oracle.fq(pos, per_atom, qbin)
+    g_want = oracle.grad_fq(pos, per_atom, qbin)
+    f_want = 10.0 * oracle.wrap_grad_rw(g_want[:, :, q_first:], f_want_fq[q_first:], target)
+    e_want = 10.0 * oracle.get_rw(target, f_want_fq[q_first:])[0]
+    print("nq=%d: energy %.3e, forces %.2e" % (nq, abs(e - e_want), rel(f, f_want)))
+    assert abs(e - e_want) < 1e-5
+    assert rel(f, f_want) < TOL

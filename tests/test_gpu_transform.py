@@ -26,14 +26,18 @@ def test_pdf_matches_fft_path(ctx, qmin):
     assert rel(got, want) < 1e-5
 
 
-def test_grad_pdf_rows(ctx):
+@pytest.mark.parametrize("n_atoms,dtype", [(37, np.float64), (342, np.float64),
+                                           (1111, np.float64), (1111, np.float32)])
+def test_grad_pdf_rows(ctx, n_atoms, dtype):
+    # 111 rows take the 64 x 64 tile product; 1,026 and 3,333 rows (at least
+    # 1,024, not multiples of the tile) the 128 x 128 XCD-mapped one
     exp, pq = oracle.make_experiment({"qmin": 0.3, "rmax": 20.0})
     nq = oracle.qmax_bin(exp["qmax"], pq)
     r = oracle.r_grid(exp)
     rs = np.random.RandomState(2)
-    g = rs.randn(37, 3, nq)
+    g = rs.randn(n_atoms, 3, nq).astype(dtype).astype(np.float64)
     ctx.set_transform(exp["rstep"], pq, r, exp["qmin"], nq)
-    got = ctx.grad_pdf(g, len(r))
+    got = ctx.grad_pdf(g.astype(dtype), len(r))
     want = oracle.grad_pdf(g.copy(), exp["rstep"], pq, r, exp["qmin"])
     assert got.shape == want.shape
     assert rel(got, want) < 1e-5
