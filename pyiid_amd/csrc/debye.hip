@@ -263,11 +263,9 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
     asm volatile("" : "+v"(qv[m]));
   }
 
-  float fi[WEIGHTED ? M : 1];
+  // Weighted: the pair factor f_i(Q) f_j(Q) is split; the loop sums f_j(Q)
+  // times the pair terms and f_i(Q) multiplies the lane's sums at the end.
   if constexpr (WEIGHTED) {
-    const int ei = static_cast<int>(pi.w);
-#pragma unroll
-    for (int m = 0; m < M; ++m) fi[m] = (q0 + m < nq) ? ftab[(size_t)ei * nq + q0 + m] : 0.0f;
     const int qb = blockIdx.y * kWaves * M;
     for (int t = threadIdx.x; t < kMaxElem * kWaves * M; t += blockDim.x) {
       const int e = t / (kWaves * M), qq = t % (kWaves * M);
@@ -343,9 +341,8 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
 #pragma unroll
               for (int e = 0; e < 4; ++e) {
                 const int m = m4 * 4 + e;
-                const float ff = fi[m] * fj[e];
-                const float t = fmaf(qv[m], cs, -w) * ff;
-                aF[m] = fmaf(w, ff, aF[m]);
+                const float t = fmaf(qv[m], cs, -w) * fj[e];
+                aF[m] = fmaf(w, fj[e], aF[m]);
                 ax[m] = fmaf(t, dpx, ax[m]);
                 ay[m] = fmaf(t, dpy, ay[m]);
                 az[m] = fmaf(t, dpz, az[m]);
@@ -384,6 +381,18 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
         }
       }
       __syncthreads();
+    }
+  }
+
+  if constexpr (WEIGHTED) {
+    const float* fti = ftab + static_cast<size_t>(pi.w) * nq;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const float f = (q0 + m < nq) ? fti[q0 + m] : 0.0f;
+      aF[m] *= f;
+      ax[m] *= f;
+      ay[m] *= f;
+      az[m] *= f;
     }
   }
 
@@ -444,11 +453,9 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
   const bool lane_ok = i < it.i_end;
   const AtomRec pi = pos[lane_ok ? i : it.i_base];
 
-  float fi[WEIGHTED ? M : 1];
+  // Weighted: the loop sums f_j(Q) omega; f_i(Q) multiplies the lane's sums
+  // at the end.
   if constexpr (WEIGHTED) {
-    const int ei = static_cast<int>(pi.w);
-#pragma unroll
-    for (int m = 0; m < M; ++m) fi[m] = (q0 + m < nq) ? ftab[(size_t)ei * nq + q0 + m] : 0.0f;
     const int qb = blockIdx.y * kWaves * M;
     for (int t = threadIdx.x; t < kMaxElem * kWaves * M; t += blockDim.x) {
       const int e = t / (kWaves * M), qq = t % (kWaves * M);
@@ -525,7 +532,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                   const int m = m4 * 4 + e;
-                  acc[m] = fmaf(w0, fi[m] * fj[e], acc[m]);
+                  acc[m] = fmaf(w0, fj[e], acc[m]);
                   float w2 = fmaf(k2hi, w1, -w0);
                   if (e == 3) w2 = fmaf(lo4, w1, w2);
                   w0 = w1;
@@ -560,7 +567,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
               for (int e = 0; e < 4; ++e) {
                 const int m = m4 * 4 + e;
-                const float ff = (e & 1) ? fi[m] * fj[e] * sgn : fi[m] * fj[e];
+                const float ff = (e & 1) ? fj[e] * sgn : fj[e];
                 acc[m] = fmaf(y, ff, acc[m]);
                 y += dd;
                 dd = fmaf(-eps, y, dd);
@@ -583,6 +590,11 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
     }
   }
 
+  if constexpr (WEIGHTED) {
+    const float* fti = ftab + static_cast<size_t>(pi.w) * nq;
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] *= (q0 + m < nq) ? fti[q0 + m] : 0.0f;
+  }
   const double mine = lane_reduce_f64<M>(acc, smem, wave, lane);
   if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine;
 }
