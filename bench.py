@@ -166,7 +166,8 @@ def kernel_suite(reps=3):
     """Each hot kernel on its own workload, one warm-up and `reps` timed
     launches (HIP events on the context stream), for profiling runs:
     F-only and pair stage on the 6,699-atom sphere (1,250 bins; all 22.4 M
-    pairs), F-only and fused F + gradient on a 20,000-atom sphere, the transform GEMM on its grad-G rows (3 N x 330 -> 4,000 r
+    pairs), F-only and fused F + gradient on a 20,113-atom Au sphere (1,250 bins) and on
+    the 20,000-atom Pt/Ni alloy (1,500 bins), the transform GEMM on its grad-G rows (3 N x 330 -> 4,000 r
     points), contracted forces and learned-integrator forces on 64 chains of
     5,000 atoms."""
     import torch
@@ -221,6 +222,27 @@ def kernel_suite(reps=3):
                                     "tflops_alg": FLOP_PER_PAIR_Q * p20 * NQ / (ms * 1e-3) / 1e12}
     del g20, f20
     c20.close()
+    # the 20,000-atom Pt/Ni random alloy (50/50, Cromer-Mann per element), 1,500 bins
+    posa = structures.jitter(structures.fcc_nearest(20000, a=3.724), SIGMA, SEED).astype(np.float32)
+    nums = structures.random_alloy(20000, 78, 28, seed=SEED)
+    ta, ia, _ = element_table(nums, QBIN, 1500)
+    ca = _lib.Context(0)
+    ca.set_positions(posa)
+    ca.set_elements(ia)
+    ca.set_scatter(_lib.SLOT_FQ, ta)
+    pa = 20000 * 19999 // 2
+    ga = torch.empty((20000, 3, 1500), dtype=torch.float32, device="cuda")
+    fa = torch.empty(1500, dtype=torch.float32, device="cuda")
+    ms = timed(ca, _lib.KERNEL_FQ, lambda: ca.fq(_lib.SLOT_FQ, QBIN, 1500, out=fa.data_ptr(),
+                                                 flags=dev))
+    out["fq_kernel_alloy20k"] = {"ms": ms, "pair_q_per_s": pa * 1500 / (ms * 1e-3)}
+    ms = timed(ca, _lib.KERNEL_FUSED, lambda: _lib.check(ca.lib.pyiid_grad_fq(
+        ca.handle, _lib.SLOT_FQ, ctypes.c_double(QBIN), ctypes.c_void_p(ga.data_ptr()),
+        ctypes.c_void_p(fa.data_ptr()), dev)))
+    out["fused_grad_kernel_alloy20k"] = {"ms": ms, "pair_q_per_s": pa * 1500 / (ms * 1e-3),
+                                         "tflops_alg": FLOP_PER_PAIR_Q * pa * 1500 / (ms * 1e-3) / 1e12}
+    del ga, fa
+    ca.close()
     d = torch.empty((pairs, 3), dtype=torch.float32, device="cuda")
     r = torch.empty(pairs, dtype=torch.float32, device="cuda")
     ms = timed(ctx, _lib.KERNEL_PAIRS, lambda: _lib.check(ctx.lib.pyiid_pair_geometry(

@@ -612,9 +612,11 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 // with S = A - B / r: the same quantity as sum_q u_q grad(i, c, q) of the
 // fused kernel (t = Q cos - sin / r there), without per-bin accumulators.
 //   part[work][side][64][3]  (side 0: I tile, 1: J tile; work = item * groups + g)
-// The phasor is re-anchored from an exact sincos every kPfChunk bins.
+// The recurrences restart every kPfChunk bins from an anchor: an FP64-reduced
+// sincos every kPfAnchor chunks, an FP32 rotation by kPfChunk dtheta between.
 // ---------------------------------------------------------------------------
 constexpr int kPfChunk = 32;
+constexpr int kPfAnchor = 4;  // chunks per FP64-reduced anchor
 
 template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void pair_force_kernel(
@@ -677,9 +679,20 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     const float* tj = tab;
     if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
+    // anchors: reduced in FP64 every kPfAnchor chunks, rotated in FP32 by
+    // kPfChunk dtheta in between (at most kPfAnchor - 1 rotations)
+    float rs = 0.f, rc = 1.f;
+    if (span > kPfChunk) sincos_red(static_cast<double>(kPfChunk) * dth, rs, rc);
+    float an_s = 0.f, an_c = 1.f;
     for (int c0 = 0; c0 < span; c0 += kPfChunk) {
-      float sn, cs;
-      sincos_red(static_cast<double>(q_lo + c0) * dth, sn, cs);
+      if ((c0 / kPfChunk) % kPfAnchor == 0) {
+        sincos_red(static_cast<double>(q_lo + c0) * dth, an_s, an_c);
+      } else {
+        const float s2 = fmaf(an_s, rc, an_c * rs);
+        an_c = fmaf(an_c, rc, -an_s * rs);
+        an_s = s2;
+      }
+      float sn = an_s, cs = an_c;
       // sin and cos follow the three-term recurrence (one FMA each per bin)
       // with 2 cos(dtheta) = k2 + lo, the low part added every fourth bin:
       // exact frequency on average, bounded error for every dtheta (close
