@@ -161,3 +161,49 @@ def test_contracted_forces_both_kernels(ctx, qbin, nq):
     print("nq=%d: energy %.3e, forces %.2e" % (nq, abs(e - e_want), rel(f, f_want)))
     assert abs(e - e_want) < 1e-5
     assert rel(f, f_want) < TOL
+
+
+def test_device_pointer_mode_matches_host_mode(ctx):
+    """PYIID_DEVICE (inputs and outputs as device pointers, ordered on the
+    context stream; what bench.py and torch callers use) gives the same
+    numbers as host mode: positions uploaded from a device tensor, F, the
+    float32 and float64 gradients, G and grad G."""
+    import torch
+    dev = _lib.PYIID_DEVICE
+    pos = structures.jitter(structures.fcc_sphere(9.0), 0.05, seed=8).astype(np.float32)
+    n, qbin, nq = len(pos), 0.02, 1250
+    load(ctx, pos, np.full(n, 79), qbin, nq)
+    f_host = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    fq_host = np.empty(nq, np.float32)
+    g_host = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_host)
+    g64_host = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, f64=True)
+
+    ctx.set_positions(torch.from_numpy(pos).cuda(), flags=dev)
+    f_dev = torch.empty(nq, dtype=torch.float32, device="cuda")
+    ctx.fq(_lib.SLOT_FQ, qbin, nq, out=f_dev, flags=dev)
+    g_dev = torch.empty((n, 3, nq), dtype=torch.float32, device="cuda")
+    fq_dev = torch.empty(nq, dtype=torch.float32, device="cuda")
+    ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, out=g_dev, fq_out=fq_dev, flags=dev)
+    g64_dev = torch.empty((n, 3, nq), dtype=torch.float64, device="cuda")
+    ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, out=g64_dev, flags=dev | _lib.PYIID_F64)
+    ctx.synchronize()
+    np.testing.assert_array_equal(f_dev.cpu().numpy(), f_host)
+    np.testing.assert_array_equal(fq_dev.cpu().numpy(), fq_host)
+    np.testing.assert_array_equal(g_dev.cpu().numpy(), g_host)
+    np.testing.assert_array_equal(g64_dev.cpu().numpy(), g64_host)
+
+    exp, pq = oracle.make_experiment({"qmin": 0.5, "rmax": 20.0})
+    nqp = oracle.qmax_bin(exp["qmax"], pq)
+    r = oracle.r_grid(exp)
+    ctx.set_transform(exp["rstep"], pq, r, exp["qmin"], nqp)
+    fq_in = np.random.RandomState(3).randn(nqp).astype(np.float32)
+    gr_host = ctx.pdf(fq_in, len(r))
+    gr_dev = torch.empty(len(r), dtype=torch.float64, device="cuda")
+    ctx.pdf(torch.from_numpy(fq_in).cuda(), len(r), out=gr_dev, flags=dev)
+    gq = np.random.RandomState(4).randn(n, 3, nqp).astype(np.float32)
+    gg_host = ctx.grad_pdf(gq, len(r))
+    gg_dev = torch.empty((n, 3, len(r)), dtype=torch.float64, device="cuda")
+    ctx.grad_pdf(torch.from_numpy(gq).cuda(), len(r), out=gg_dev, flags=dev)
+    ctx.synchronize()
+    np.testing.assert_array_equal(gr_dev.cpu().numpy(), gr_host)
+    np.testing.assert_array_equal(gg_dev.cpu().numpy(), gg_host)
