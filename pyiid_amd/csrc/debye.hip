@@ -167,10 +167,13 @@ constexpr int kGeoFields = 7 + 2 * kWaves;
 constexpr int kGeoBytes = kGeoJ * 64 * kGeoFields * 4;
 constexpr int kFusedSmemBytes =
     kSmemBytes > kJTile * 32 + kGeoBytes ? kSmemBytes : kJTile * 32 + kGeoBytes;
-// F-only kernel: the shared setup is dtheta (FP64) and 1/r, sin(dtheta),
-// 2 cos(dtheta) as hi + lo, eps of d and (-1)^k (FP32), dtheta = k pi + d.
-constexpr int kFqGeoFields = 6;
-constexpr int kFqGeoBytes = kGeoJ * 64 * (8 + kFqGeoFields * 4);
+// F-only kernel: rounds of kFqGeoJ j atoms share the per-pair setup (FP32,
+// [field][k][lane]): 1/r, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, eps of d,
+// (-1)^k (dtheta = k pi + d), and the anchor sin, cos(Q0 r) of every wave's Q
+// chunk.  Rounds of 8 keep the block at 37 KB of LDS.
+constexpr int kFqGeoJ = 8;
+constexpr int kFqGeoFields = 6 + 2 * kWaves;
+constexpr int kFqGeoBytes = kFqGeoJ * 64 * kFqGeoFields * 4;
 constexpr int kFqSmemBytes = kWaves * kRedM * kRedRow * 4 > kJTile * 32 + kFqGeoBytes
                                  ? kWaves * kRedM * kRedRow * 4
                                  : kJTile * 32 + kFqGeoBytes;
@@ -440,9 +443,8 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
                                                  double* __restrict__ fpart) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFqSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
-  double* geo_dth = reinterpret_cast<double*>(smem + kJTile * 32);
-  float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32 + kGeoJ * 64 * 8);
-  constexpr int kGS = kGeoJ * 64;
+  float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
+  constexpr int kGS = kFqGeoJ * 64;
   __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
   const Item it = items[blockIdx.x];
@@ -467,14 +469,15 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 #pragma unroll
   for (int m = 0; m < M; ++m) acc[m] = 0.0f;
 
-  const double q0d = static_cast<double>(q0);
+  // first bin of the block (its waves take consecutive chunks of M bins)
+  const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
   for (int jt = it.jbeg; jt < it.jend; jt += kJTile) {
     __syncthreads();
     if (jt + (int)threadIdx.x < it.jend) tile[threadIdx.x] = pos[jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, it.jend - jt);
-    for (int js = 0; js < cnt; js += kGeoJ) {
-      const int kk = min(kGeoJ, cnt - js);
+    for (int js = 0; js < cnt; js += kFqGeoJ) {
+      const int kk = min(kFqGeoJ, cnt - js);
       // setup shared by the block's waves (same atoms, different Q chunks)
       for (int k = wave; k < kk; k += kWaves) {
         const PairGeom g = pair_geom(pi, tile[js + k], lane_ok && (jt + js + k < i));
@@ -485,7 +488,6 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
         sincos_poly(static_cast<float>(0.5 * d), sh, ch);
         const float sgn = k_odd ? -1.0f : 1.0f;
         const int o = k * 64 + lane;
-        geo_dth[o] = dth;
         geo_f[o] = g.inv_r;
         // sin(dtheta) = (-1)^k sin d; 2 cos(dtheta) = (-1)^k (2 - 4 sin^2(d/2))
         // in FP64, kept as hi + lo so that the three-term recurrence runs at
@@ -499,17 +501,28 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
         geo_f[o + 3 * kGS] = 4.0f * static_cast<float>(k2 - static_cast<double>(k2hi));
         geo_f[o + 4 * kGS] = 4.0f * sh * sh;
         geo_f[o + 5 * kGS] = sgn;
+        // the anchors sin, cos(Q0 r) of the block's kWaves Q chunks: the
+        // first reduced in FP64, the others by FP32 rotations by M dtheta
+        float as, ac, rs, rc;
+        sincos_red(qbase * dth, as, ac);
+        sincos_red(static_cast<double>(M) * dth, rs, rc);
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+          geo_f[o + (6 + 2 * w) * kGS] = as;
+          geo_f[o + (7 + 2 * w) * kGS] = ac;
+          const float as2 = fmaf(as, rc, ac * rs);
+          ac = fmaf(ac, rc, -as * rs);
+          as = as2;
+        }
       }
       __syncthreads();
       if (q0 < nq) {  // wave-uniform
         for (int k = 0; k < kk; ++k) {
           const int o = k * 64 + lane;
-          const double dth = geo_dth[o];
           const float inv_r = geo_f[o], s1 = geo_f[o + kGS], k2hi = geo_f[o + 2 * kGS],
                       lo4 = geo_f[o + 3 * kGS], eps = geo_f[o + 4 * kGS],
                       sgn = geo_f[o + 5 * kGS];
-          float sn, cs;
-          sincos_red(q0d * dth, sn, cs);
+          const float sn = geo_f[o + (6 + 2 * wave) * kGS], cs = geo_f[o + (7 + 2 * wave) * kGS];
           float w0 = sn * inv_r;
           // When every kept pair of the wave has |d| >= kChebMin the
           // three-term form w2 = 2 cos(dtheta) w1 - w0 with the two-float
