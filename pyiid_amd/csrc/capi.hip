@@ -486,7 +486,8 @@ int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, bool tune = t
   if (tune && !u) CHK(tune_fused(c, slot, qbin));
   CHK(ensure_fused_items(c, nq));
   const ItemSet& s = c->fused_items;
-  const int n_qchunk = ((nq + kWaves * c->grad_m - 1) / (kWaves * c->grad_m)) * kWaves;
+  const int n_qchunk =
+      ((nq + kFusedWaves * c->grad_m - 1) / (kFusedWaves * c->grad_m)) * kFusedWaves;
   CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, s.n_items) * nq));
   if (u) {
     CHK(c->forcepart.reserve(sizeof(float) * (size_t)std::max(1, s.n_items) * n_qchunk * 64 * 3));
@@ -700,7 +701,8 @@ int contract_forces(pyiid_ctx* c, int slot, double qbin, double scale, double* o
     return PYIID_OK;
   }
   CHK(run_fused(c, slot, qbin, c->u.as<float>()));
-  const int n_qchunk = ((nq + kWaves * c->grad_m - 1) / (kWaves * c->grad_m)) * kWaves;
+  const int n_qchunk =
+      ((nq + kFusedWaves * c->grad_m - 1) / (kFusedWaves * c->grad_m)) * kFusedWaves;
   HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
                              c->fused_items.n_ranges, n_qchunk, scale, out));
   return PYIID_OK;

@@ -163,10 +163,10 @@ constexpr int kSmemBytes = cmax3(kWaves * kRedM * kRedRow * 4, kJTile * 32,
 // [field][k][lane]): 1/r, (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi and
 // 4 lo, and the anchor sin, cos(Q0 r) of every wave's Q chunk.
 constexpr int kGeoJ = 16;
-constexpr int kGeoFields = 7 + 2 * kWaves;
+constexpr int kGeoFields = 7 + 2 * kFusedWaves;
 constexpr int kGeoBytes = kGeoJ * 64 * kGeoFields * 4;
-constexpr int kFusedSmemBytes =
-    kSmemBytes > kJTile * 32 + kGeoBytes ? kSmemBytes : kJTile * 32 + kGeoBytes;
+constexpr int kFusedSmemBytes = cmax3(kFusedWaves * kRedM * kRedRow * 4,
+                                      kFusedWaves * 64 * kStoreRow * 4, kJTile * 32 + kGeoBytes);
 // F-only kernel: rounds of kFqGeoJ j atoms share the per-pair setup (FP32,
 // [field][k][lane]): 1/r, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, eps of d,
 // (-1)^k (dtheta = k pi + d), and the anchor sin, cos(Q0 r) of every wave's Q
@@ -229,14 +229,14 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 
 // ---------------------------------------------------------------------------
 // Fused F + gradient kernel (ordered pairs).
-//   grid: x = work item (i tile x j range), y = Q group (kWaves waves x M bins)
+//   grid: x = work item (i tile x j range), y = Q group (kFusedWaves waves x M bins)
 //   gpart[split][i][c][q]  float  sum_j t_ij(q) (q_i - q_j)_c / r^2 (ff-weighted)
 //   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
 //   CONTRACT: instead of gpart, force_part[item][qchunk][lane][c] =
 //             sum_q u[chain][q] * gpart(i, c, q)
 // ---------------------------------------------------------------------------
 template <int M, bool WEIGHTED, bool CONTRACT>
-__global__ __launch_bounds__(256) void fused_grad_kernel(
+__global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
     const AtomRec* __restrict__ pos, int n_total, const float* __restrict__ ftab, int n_elem,
     int nq, double dq, const Item* __restrict__ items, float* __restrict__ gpart,
     double* __restrict__ fpart, const float* __restrict__ u, float* __restrict__ force_part) {
@@ -244,13 +244,13 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
   constexpr int kGS = kGeoJ * 64;  // field stride in geo_f
-  __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
+  __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kFusedWaves * M : 1];
 
   const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = it.i_base + lane;
-  const int qchunk = blockIdx.y * kWaves + wave;
+  const int qchunk = blockIdx.y * kFusedWaves + wave;
   const int q0 = qchunk * M;
   const int j_begin = it.jbeg;
   const int j_end = it.jend;
@@ -269,9 +269,9 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
   // Weighted: the pair factor f_i(Q) f_j(Q) is split; the loop sums f_j(Q)
   // times the pair terms and f_i(Q) multiplies the lane's sums at the end.
   if constexpr (WEIGHTED) {
-    const int qb = blockIdx.y * kWaves * M;
-    for (int t = threadIdx.x; t < kMaxElem * kWaves * M; t += blockDim.x) {
-      const int e = t / (kWaves * M), qq = t % (kWaves * M);
+    const int qb = blockIdx.y * kFusedWaves * M;
+    for (int t = threadIdx.x; t < kMaxElem * kFusedWaves * M; t += blockDim.x) {
+      const int e = t / (kFusedWaves * M), qq = t % (kFusedWaves * M);
       fl[e][qq] = (e < n_elem && qb + qq < nq) ? ftab[(size_t)e * nq + qb + qq] : 0.0f;
     }
   }
@@ -281,16 +281,17 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
   for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = 0.0f;
 
   // first bin of the block (its waves take consecutive chunks of M bins)
-  const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
+  const double qbase = static_cast<double>(blockIdx.y * kFusedWaves * M);
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
-    if (jt + (int)threadIdx.x < j_end) tile[threadIdx.x] = pos[jt + threadIdx.x];
+    if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end)
+      tile[threadIdx.x] = pos[jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, j_end - jt);
     for (int js = 0; js < cnt; js += kGeoJ) {
       const int kk = min(kGeoJ, cnt - js);
       // shared setup: wave w forms the pairs of j = js + w, js + w + 4, ...
-      for (int k = wave; k < kk; k += kWaves) {
+      for (int k = wave; k < kk; k += kFusedWaves) {
         const PairGeom g = pair_geom(pi, tile[js + k], lane_ok);
         const double dth = dq * g.r;
         float s1, k2hi, k2lo;
@@ -303,13 +304,13 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
         geo_f[o + 4 * kGS] = s1;
         geo_f[o + 5 * kGS] = k2hi;
         geo_f[o + 6 * kGS] = 4.0f * k2lo;
-        // the anchors sin, cos(Q0 r) of the block's kWaves Q chunks: the
+        // the anchors sin, cos(Q0 r) of the block's kFusedWaves Q chunks: the
         // first reduced in FP64, the others by FP32 rotations by M dtheta
         float as, ac, rs, rc;
         sincos_red(qbase * dth, as, ac);
         sincos_red(static_cast<double>(M) * dth, rs, rc);
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
+        for (int w = 0; w < kFusedWaves; ++w) {
           geo_f[o + (7 + 2 * w) * kGS] = as;
           geo_f[o + (8 + 2 * w) * kGS] = ac;
           const float as2 = fmaf(as, rc, ac * rs);
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) void fused_grad_kernel(
       fy = fmaf(um, ay[m], fy);
       fz = fmaf(um, az[m], fz);
     }
-    const size_t p = (size_t)blockIdx.x * (gridDim.y * kWaves) + qchunk;
+    const size_t p = (size_t)blockIdx.x * (gridDim.y * kFusedWaves) + qchunk;
     float* dst = force_part + (p * 64 + lane) * 3;
     dst[0] = fx;
     dst[1] = fy;
@@ -909,8 +910,8 @@ hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, int n_total, const
                           int n_elem, int nq, double dq, const Item* items, int n_items,
                           bool weighted, float* gpart, double* fpart, const float* u,
                           float* force_part) {
-  dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
-  dim3 block(256);
+  dim3 grid(n_items, (nq + kFusedWaves * M - 1) / (kFusedWaves * M));
+  dim3 block(64 * kFusedWaves);
   const bool contract = (u != nullptr);
   if (weighted) {
     if (contract)

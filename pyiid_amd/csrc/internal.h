@@ -17,7 +17,8 @@ struct alignas(32) AtomRec {
 // Q bins per wave in the fused F + gradient kernel and in the F-only kernel.
 constexpr int kGradM = 32;
 constexpr int kFqM = 64;
-constexpr int kWaves = 4;      // waves per block in the Debye kernels
+constexpr int kWaves = 4;      // waves per block in the F-only and contracted-force kernels
+constexpr int kFusedWaves = 8; // waves per block in the fused F + gradient kernel
 constexpr int kJTile = 256;    // atoms staged in LDS per j tile
 constexpr int kMaxElem = 8;    // element-table rows staged in LDS (weighted kernels)
 
@@ -39,7 +40,7 @@ struct Range {
 // Ordered-pair F + gradient, m = 16 or 32 Q bins per wave.  Non-contract:
 // gpart[item.split][i][c][q] (float) for i in the item's tile.  Contract
 // (u != null): force_part[item][qchunk][64][3] = sum_q u[chain][q] * grad(i, c, q),
-// qchunk over ceil(nq / (kWaves m)) * kWaves.  fpart[item][q]: F partial (double).
+// qchunk over ceil(nq / (kFusedWaves m)) * kFusedWaves.  fpart[item][q]: F partial (double).
 hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, int n_total,
                              const float* ftab, int n_elem, int nq, double dq, const Item* items,
                              int n_items, bool weighted, float* gpart, double* fpart,
