@@ -308,7 +308,7 @@ int ensure_norm(pyiid_ctx* c, int slot) {
   return PYIID_OK;
 }
 
-int upload_items(pyiid_ctx* c, ItemSet& s, const std::vector<Item>& v, const std::vector<Range>& r) {
+int upload_items(pyiid_ctx*, ItemSet& s, const std::vector<Item>& v, const std::vector<Range>& r) {
   CHK(s.items.reserve(sizeof(Item) * std::max<size_t>(1, v.size())));
   CHK(s.ranges.reserve(sizeof(Range) * std::max<size_t>(1, r.size())));
   if (!v.empty())

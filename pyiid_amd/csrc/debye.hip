@@ -32,10 +32,6 @@ namespace {
 
 using f4v = float __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float uniform_f(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
 // sin and cos of |rf| <= pi/4: Taylor series, sin through r^9 and cos
 // through r^10 (truncation < 2e-9).
 __device__ __forceinline__ void sincos_poly(float rf, float& s, float& c) {
@@ -143,11 +139,10 @@ __device__ __forceinline__ PairGeom pair_geom(const AtomRec& pi, const AtomRec& 
 }
 
 
-// Smallest distance |d| of the per-bin phase step from a multiple of pi for
-// the three-term recurrence of the F-only kernel (64 bins per anchor); closer
-// steps take the Reinsch form.  The kernel tests
-// eps(d) = 4 sin^2(d / 2) >= 4 sin^2(kChebMin / 2).
-constexpr double kChebMin = 0.2;
+// Smallest distance |d| = 0.2 of the per-bin phase step from a multiple of pi
+// for the three-term recurrence of the F-only kernel (64 bins per anchor);
+// closer steps take the Reinsch form.  The kernel tests
+// eps(d) = 4 sin^2(d / 2) >= 4 sin^2(0.1).
 constexpr float kChebEpsMin = 0.039866844f;
 
 // Shared memory of the Debye kernels: the j tile during the pair loop, then
@@ -156,8 +151,6 @@ constexpr int kRedRow = 65;  // 64 lanes + 1 pad (bank spread)
 constexpr int kRedM = 16;    // Q bins reduced per pass
 constexpr int kStoreRow = kGradM + 4;  // 16-byte aligned rows for the store transpose
 constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-constexpr int kSmemBytes = cmax3(kWaves * kRedM * kRedRow * 4, kJTile * 32,
-                                 kWaves * 64 * kStoreRow * 4);
 // Fused kernel: per round of kGeoJ j atoms the block's waves (same atoms,
 // different Q chunks) share the per-pair setup through LDS (FP32,
 // [field][k][lane]): 1/r, (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi and
@@ -525,7 +518,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
                       sgn = geo_f[o + 5 * kGS];
           const float sn = geo_f[o + (6 + 2 * wave) * kGS], cs = geo_f[o + (7 + 2 * wave) * kGS];
           float w0 = sn * inv_r;
-          // When every kept pair of the wave has |d| >= kChebMin the
+          // When every kept pair of the wave has |d| >= 0.2 the
           // three-term form w2 = 2 cos(dtheta) w1 - w0 with the two-float
           // coefficient (one FMA per bin, the low part added every fourth bin:
           // exact frequency on average, rounding amplified by at most

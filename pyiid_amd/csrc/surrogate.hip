@@ -155,7 +155,7 @@ __device__ __forceinline__ void silu(float z, float& h, float& dh) {
 // result layout to the A layout through a per-wave LDS tile.
 // Output: c[i][m] = dE/dD[i][m]; e_atom[i] (nullable).  w3 row of chain b at
 // w3 + b * w3_stride (stride 0: one row for all chains).
-__global__ __launch_bounds__(256) void mlp_kernel(const float* __restrict__ d, int n,
+__global__ __launch_bounds__(256) void mlp_kernel(const float* __restrict__ d,
                                                   const Item* __restrict__ tiles,
                                                   const Net* __restrict__ net,
                                                   const unsigned short* __restrict__ w3,
@@ -452,11 +452,11 @@ hipError_t launch_descriptors(hipStream_t st, const AtomRec* pos, int n, const i
   return hipGetLastError();
 }
 
-hipError_t launch_mlp(hipStream_t st, const float* d, int n, const Item* tiles, int n_tiles,
+hipError_t launch_mlp(hipStream_t st, const float* d, int, const Item* tiles, int n_tiles,
                       const void* net, const unsigned short* w3, int w3_stride, float* c,
                       float* e_atom) {
   if (n_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(mlp_kernel, dim3(n_tiles), dim3(256), 0, st, d, n, tiles,
+  hipLaunchKernelGGL(mlp_kernel, dim3(n_tiles), dim3(256), 0, st, d, tiles,
                      static_cast<const Net*>(net), w3, w3_stride, c, e_atom);
   return hipGetLastError();
 }

@@ -17,8 +17,6 @@ forces until the next fit.
 """
 import numpy as np
 
-from . import _lib
-
 RC = 6.0
 SKIN = 0.5
 MU0 = 2.0
