@@ -179,3 +179,42 @@ def test_graph_replay_equals_direct_and_lowers_rw():
     np.testing.assert_allclose(e1, e2, rtol=1e-12)
     assert np.all(e1[-1] < e1[0])         # refinement towards the target lowers Rw
     assert 0 < a1.mean() <= 1
+
+
+@pytest.mark.parametrize("tiny", [1, 0])
+def test_batch_with_degenerate_chain(tiny):
+    """A chain of one atom (no pairs) or of none in the middle of a batch:
+    its G is zero, so Rw is 0/0 = NaN as in the reference's get_rw
+    (master_kernel.py:206-236); the other chains' energies and forces are
+    those of the batch without it (to the rounding of launch shapes the
+    autotuner may pick differently for the two batches)."""
+    e, pq = S.make_experiment(dict(EXP))
+    nq = S.qmax_bin(e["qmax"], pq)
+    r = S.r_grid(e)
+    a = structures.jitter(structures.fcc_nearest(13), 0.1, seed=1).astype(np.float32)
+    b = structures.jitter(structures.fcc_nearest(19), 0.1, seed=2).astype(np.float32)
+    t = np.zeros((tiny, 3), np.float32)
+    table, _, _ = element_table(np.array([79]), pq, nq)
+    gobs = np.stack([S.get_pdf_at_qmin(S.fq(p, table[np.zeros(len(p), int)], pq), e["rstep"], pq,
+                                       r, e["qmin"]) for p in (a, b)])
+    gobs = gobs * 1.01
+
+    def run(chains, targets):
+        ctx = _lib.Context(0)
+        ctx.set_batch(chains)
+        n = sum(len(c) for c in chains)
+        ctx.set_elements(np.zeros(n, np.int32))
+        ctx.set_scatter(_lib.SLOT_PDF, table)
+        ctx.set_transform(e["rstep"], pq, r, e["qmin"], nq)
+        en, forces = ctx.pdf_energy_forces(_lib.SLOT_PDF, pq, targets, _lib.POT_RW, 2.0, n)
+        ctx.close()
+        return en, forces
+
+    en2, f2 = run([a, b], gobs)
+    en3, f3 = run([a, t, b], np.stack([gobs[0], gobs[0], gobs[1]]))
+    assert np.isnan(en3[1])
+    assert np.all(np.isfinite(en3[[0, 2]]))
+    np.testing.assert_allclose(en3[[0, 2]], en2, rtol=1e-9)
+    scale = np.max(np.abs(f2))
+    np.testing.assert_allclose(f3[:13], f2[:13], rtol=0, atol=1e-6 * scale)
+    np.testing.assert_allclose(f3[13 + tiny:], f2[13:], rtol=0, atol=1e-6 * scale)
