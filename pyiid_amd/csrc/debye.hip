@@ -485,15 +485,16 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
         geo_f[o] = g.inv_r;
         // sin(dtheta) = (-1)^k sin d; 2 cos(dtheta) = (-1)^k (2 - 4 sin^2(d/2))
         // in FP64, kept as hi + lo so that the three-term recurrence runs at
-        // the exact frequency; eps = 4 sin^2(d/2) keeps its relative accuracy
-        // for small |d|.
+        // the exact frequency; eps = 4 sin^2(d/2), rounded once from FP64,
+        // keeps its relative accuracy for every |d| (a Reinsch wave runs all
+        // its lanes in that form, whatever their step).
         const double shd = sin_poly_f64(0.5 * d);
         const double k2 = (k_odd ? -1.0 : 1.0) * fma(-4.0 * shd, shd, 2.0);
         const float k2hi = static_cast<float>(k2);
         geo_f[o + kGS] = sgn * 2.0f * sh * ch;
         geo_f[o + 2 * kGS] = k2hi;
         geo_f[o + 3 * kGS] = 4.0f * static_cast<float>(k2 - static_cast<double>(k2hi));
-        geo_f[o + 4 * kGS] = 4.0f * sh * sh;
+        geo_f[o + 4 * kGS] = static_cast<float>(4.0 * shd * shd);
         geo_f[o + 5 * kGS] = sgn;
         // the anchors sin, cos(Q0 r) of the block's kWaves Q chunks: the
         // first reduced in FP64, the others by FP32 rotations by M dtheta

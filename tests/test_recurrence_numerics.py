@@ -102,3 +102,10 @@ def test_reinsch_form_near_multiples_of_pi(dth):
     # the F-only kernel's form for steps within 0.2 of k pi: a few ulp after
     # 64 steps, where the two-float three-term form reaches ~120
     assert worst(reinsch, dth, 64) < 10
+
+
+@pytest.mark.parametrize("dth", [0.5, 1.0, 1.3, np.pi / 2 - 0.01, 2.0, 4.0, 5.5])
+def test_reinsch_form_for_every_step(dth):
+    # a wave that takes the Reinsch form runs it on all its lanes, whatever
+    # their step: with eps rounded once from FP64 it stays within ~15 ulp
+    assert worst(reinsch, dth, 64) < 20
