@@ -96,6 +96,7 @@ struct Table {
 // A work-item list and its ranges, cached against the structure.
 struct ItemSet {
   Buf items, ranges;
+  Buf perm;  // int32 [n]: atom index per position (F-only items of a weighted table)
   int n_items = 0, n_ranges = 0, split = 0;
   uint64_t stamp = ~0ull;
   int key = -1;
@@ -266,9 +267,6 @@ int check_table(pyiid_ctx* c, int slot) {
   if (c->max_elem >= t.n_elem)
     return fail(PYIID_E_INVALID, "element index %d out of range for a %d-row scatter table",
                 c->max_elem, t.n_elem);
-  if (t.n_elem > 1 && t.n_elem > kMaxElem)
-    return fail(PYIID_E_INVALID, "at most %d elements are supported (got %d)", kMaxElem,
-                t.n_elem);
   return PYIID_OK;
 }
 
@@ -321,37 +319,68 @@ int upload_items(pyiid_ctx*, ItemSet& s, const std::vector<Item>& v, const std::
 }
 
 // Unordered-pair items: per chain, per 64-atom i tile, j ranges of the chain
-// below the tile's end.  Ranges: one per chain.
-int ensure_fq_items(pyiid_ctx* c) {
+// below the tile's end.  Ranges: one per chain.  Item indices are positions
+// in perm, the chain's atoms in element order (stable) for a weighted table
+// and in their own order otherwise; for a weighted table no j range crosses
+// an element boundary and item.pad is the range's element, so that the
+// kernel sums plain sin(Q r)/r and weights the sums by f_i(Q) f_j(Q) once.
+int ensure_fq_items(pyiid_ctx* c, bool weighted) {
   ItemSet& s = c->fq_items;
-  if (s.stamp == c->struct_stamp) return PYIID_OK;
+  const int key = weighted ? 1 : 0;
+  if (s.stamp == c->struct_stamp && s.key == key) return PYIID_OK;
   if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
   std::vector<Item> v;
   std::vector<Range> r;
+  std::vector<int32_t> perm(static_cast<size_t>(c->n));
+  for (int64_t a = 0; a < c->n; ++a) perm[a] = static_cast<int32_t>(a);
+  auto el = [&](int32_t a) { return c->elem_h.empty() ? 0 : c->elem_h[a]; };
   int tile = 0;
   for (int b = 0; b < c->n_chain; ++b) {
     const int64_t o = c->off[b], nb = c->off[b + 1] - o;
     const int begin = static_cast<int>(v.size());
     const int64_t jc = 256 * std::max<int64_t>(1, nb / (256 * 16));
+    // seg_end[p - o]: end of the element run holding position p
+    std::vector<int64_t> seg_end(static_cast<size_t>(nb), o + nb);
+    if (weighted) {
+      std::stable_sort(perm.begin() + o, perm.begin() + o + nb,
+                       [&](int32_t x, int32_t y) { return el(x) < el(y); });
+      for (int64_t p = o + nb - 2; p >= o; --p)
+        seg_end[p - o] = el(perm[p]) == el(perm[p + 1]) ? seg_end[p + 1 - o] : p + 1;
+    }
     for (int64_t ib = o; ib < o + nb; ib += 64, ++tile) {
       const int64_t ie = std::min(o + nb, ib + 64);
-      for (int64_t jb = o; jb < ie; jb += jc)
+      for (int64_t jb = o; jb < ie;) {
+        int64_t je = std::min(ie, jb + jc);
+        int ej = 0;
+        if (weighted) {
+          je = std::min(je, seg_end[jb - o]);
+          ej = el(perm[jb]);
+        }
         v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(jb),
-                         static_cast<int>(std::min(ie, jb + jc)), 0, b, tile, 0});
+                         static_cast<int>(je), 0, b, tile, ej});
+        jb = je;
+      }
     }
     r.push_back(Range{begin, static_cast<int>(v.size()), static_cast<int>(o),
                       static_cast<int>(o + nb)});
   }
   CHK(upload_items(c, s, v, r));
+  CHK(s.perm.reserve(sizeof(int32_t) * std::max<size_t>(1, perm.size())));
+  if (!perm.empty())
+    HIPCHK(hipMemcpy(s.perm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
   s.stamp = c->struct_stamp;
+  s.key = key;
   return PYIID_OK;
 }
 
-// Ordered-pair items: per chain, per i tile, `split` j ranges covering the
-// chain.  Every (tile, split) item exists (possibly with an empty j range) so
-// that the gradient partial buffer [split][n][3][nq] is fully written.
-// Ranges: one per i tile (for the force reduction).
-int ensure_fused_items(pyiid_ctx* c, int nq) {
+// Ordered-pair items: per chain, per i tile, the same j ranges covering the
+// chain: `split` chunks, and for a weighted table the chunks cut further at
+// element boundaries of the chain's atoms in element order (j indices are
+// positions in perm; item.pad is the range's element).  Every tile has as
+// many items as the chain with the most j ranges (padded with empty ranges)
+// so that the gradient partial buffer [split][n][3][nq], one plane per j
+// range, is fully written.  Ranges: one per i tile (for the force reduction).
+int ensure_fused_items(pyiid_ctx* c, int nq, bool weighted) {
   ItemSet& s = c->fused_items;
   int64_t n_tiles = 0, max_nb = 0;
   for (int b = 0; b < c->n_chain; ++b) {
@@ -364,22 +393,52 @@ int ensure_fused_items(pyiid_ctx* c, int nq) {
   split = std::min<int64_t>(split, 32);
   split = std::min<int64_t>(split, std::max<int64_t>(1, max_nb / 64));
   split = std::max<int64_t>(1, split);
-  if (s.stamp == c->struct_stamp && s.key == static_cast<int>(split)) return PYIID_OK;
+  const int key = static_cast<int>(split) * 2 + (weighted ? 1 : 0);
+  if (s.stamp == c->struct_stamp && s.key == key) return PYIID_OK;
   if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  std::vector<int32_t> perm(static_cast<size_t>(c->n));
+  for (int64_t a = 0; a < c->n; ++a) perm[a] = static_cast<int32_t>(a);
+  auto el = [&](int32_t a) { return c->elem_h.empty() ? 0 : c->elem_h[a]; };
+  struct JRange {
+    int64_t jb, je;
+    int elem;
+  };
+  std::vector<std::vector<JRange>> jr(static_cast<size_t>(c->n_chain));
+  size_t planes = 1;
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int64_t o = c->off[b], nb = c->off[b + 1] - o;
+    const int64_t jc = (nb + split - 1) / split;
+    if (weighted)
+      std::stable_sort(perm.begin() + o, perm.begin() + o + nb,
+                       [&](int32_t x, int32_t y) { return el(x) < el(y); });
+    for (int64_t sp = 0; sp < split; ++sp) {
+      const int64_t jb = std::min(o + nb, o + sp * jc);
+      const int64_t je = std::min(o + nb, jb + jc);
+      if (!weighted || jb == je) {
+        jr[b].push_back(JRange{jb, je, 0});
+        continue;
+      }
+      for (int64_t p = jb; p < je;) {
+        int64_t q = p + 1;
+        while (q < je && el(perm[q]) == el(perm[p])) ++q;
+        jr[b].push_back(JRange{p, q, el(perm[p])});
+        p = q;
+      }
+    }
+    planes = std::max(planes, jr[b].size());
+  }
   std::vector<Item> v;
   std::vector<Range> r;
   int tile = 0;
   for (int b = 0; b < c->n_chain; ++b) {
     const int64_t o = c->off[b], nb = c->off[b + 1] - o;
-    const int64_t jc = (nb + split - 1) / split;
     for (int64_t ib = o; ib < o + nb; ib += 64, ++tile) {
       const int64_t ie = std::min(o + nb, ib + 64);
       const int begin = static_cast<int>(v.size());
-      for (int64_t sp = 0; sp < split; ++sp) {
-        const int64_t jb = std::min(o + nb, o + sp * jc);
-        const int64_t je = std::min(o + nb, jb + jc);
-        v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(jb),
-                         static_cast<int>(je), static_cast<int>(sp), b, tile, 0});
+      for (size_t sp = 0; sp < planes; ++sp) {
+        const JRange x = sp < jr[b].size() ? jr[b][sp] : JRange{o + nb, o + nb, 0};
+        v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(x.jb),
+                         static_cast<int>(x.je), static_cast<int>(sp), b, tile, x.elem});
       }
       r.push_back(Range{begin, static_cast<int>(v.size()), static_cast<int>(ib),
                         static_cast<int>(ie)});
@@ -389,9 +448,12 @@ int ensure_fused_items(pyiid_ctx* c, int nq) {
   const int n_tile_ranges = static_cast<int>(r.size());
   r.push_back(Range{0, static_cast<int>(v.size()), 0, static_cast<int>(c->n)});
   CHK(upload_items(c, s, v, r));
+  CHK(s.perm.reserve(sizeof(int32_t) * std::max<size_t>(1, perm.size())));
+  if (!perm.empty())
+    HIPCHK(hipMemcpy(s.perm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
   s.n_ranges = n_tile_ranges;
-  s.split = static_cast<int>(split);
-  s.key = static_cast<int>(split);
+  s.split = static_cast<int>(planes);
+  s.key = key;
   s.stamp = c->struct_stamp;
   return PYIID_OK;
 }
@@ -460,13 +522,13 @@ int compute_fq(pyiid_ctx* c, int slot, double qbin) {
   CHK(c->fq32.reserve(sizeof(float) * (size_t)std::max(1, nq) * B));
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
-  CHK(ensure_fq_items(c));
+  CHK(ensure_fq_items(c, weighted_table(t)));
   CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, c->fq_items.n_items) * nq));
   const double dq = static_cast<double>(static_cast<float>(qbin));
   Timed tm(c, 1);
-  HIPCHK(launch_fq(c->stream, c->pos.as<AtomRec>(), static_cast<int>(c->n), t.tab.as<float>(),
-                   t.n_elem, nq, dq, c->fq_items.items.as<Item>(), c->fq_items.n_items,
-                   weighted_table(t), c->fpart.as<double>()));
+  HIPCHK(launch_fq(c->stream, c->pos.as<AtomRec>(), c->fq_items.perm.as<int32_t>(),
+                   t.tab.as<float>(), nq, dq, c->fq_items.items.as<Item>(),
+                   c->fq_items.n_items, weighted_table(t), c->fpart.as<double>()));
   CHK(tm.done());
   // chains without pairs have empty ranges: their F is 0
   HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), c->fq_items.ranges.as<Range>(), B,
@@ -484,7 +546,7 @@ int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, bool tune = t
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
   if (tune && !u) CHK(tune_fused(c, slot, qbin));
-  CHK(ensure_fused_items(c, nq));
+  CHK(ensure_fused_items(c, nq, weighted_table(t)));
   const ItemSet& s = c->fused_items;
   const int n_qchunk =
       ((nq + kFusedWaves * c->grad_m - 1) / (kFusedWaves * c->grad_m)) * kFusedWaves;
@@ -496,10 +558,10 @@ int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, bool tune = t
   }
   const double dq = static_cast<double>(static_cast<float>(qbin));
   Timed tm(c, 0);
-  HIPCHK(launch_fused_grad(c->stream, c->grad_m, c->pos.as<AtomRec>(), static_cast<int>(c->n),
-                           t.tab.as<float>(), t.n_elem, nq, dq, s.items.as<Item>(), s.n_items,
-                           weighted_table(t), c->gpart.as<float>(), c->fpart.as<double>(), u,
-                           c->forcepart.as<float>()));
+  HIPCHK(launch_fused_grad(c->stream, c->grad_m, c->pos.as<AtomRec>(), s.perm.as<int32_t>(),
+                           static_cast<int>(c->n), t.tab.as<float>(), nq, dq, s.items.as<Item>(),
+                           s.n_items, weighted_table(t), c->gpart.as<float>(),
+                           c->fpart.as<double>(), u, c->forcepart.as<float>()));
   CHK(tm.done());
   return PYIID_OK;
 }
@@ -1219,7 +1281,8 @@ int pyiid_destroy(pyiid_ctx* c) {
   for (Buf* b : {&c->posf, &c->elem, &c->pos, &c->counts_d, &c->natoms_d, &c->tmat, &c->gpart,
                  &c->fpart, &c->forcepart, &c->fq64, &c->fq32, &c->g64, &c->io_a, &c->io_b,
                  &c->io_c, &c->w, &c->out5, &c->u, &c->gobs, &c->fq_items.items,
-                 &c->fq_items.ranges, &c->fused_items.items, &c->fused_items.ranges, &c->hmc.mass,
+                 &c->fq_items.ranges, &c->fq_items.perm, &c->fused_items.items, &c->fused_items.ranges,
+                 &c->fused_items.perm, &c->hmc.mass,
                  &c->hmc.mom, &c->hmc.fcur, &c->hmc.fnew, &c->hmc.pos_save, &c->hmc.kin0,
                  &c->hmc.ecur, &c->hmc.ehist, &c->hmc.acchist, &c->hmc.counter, &c->hmc.chains,
                  &c->hmc.gobs, &c->sur.net, &c->sur.w3, &c->sur.onehot, &c->sur.nbr, &c->sur.cnt,

@@ -19,7 +19,9 @@
 //    Reinsch form
 //    w_{m+1} = w_m + D_{m+1}, D_{m+2} = D_{m+1} - eps w_{m+1}, eps =
 //    4 sin^2(d/2) from a half-angle sincos of d = dtheta - k pi, on the
-//    sequence (-1)^(k m) w_m.
+//    sequence (-1)^(k m) w_m.  With several elements the items run over the
+//    atoms in element order, each j range of one element, so the loop is the
+//    same and the sums take the weight f_i(Q) f_j(Q) once, at the end.
 //  * partial sums leave each wave in FP64 and are reduced across tiles in
 //    FP64 by small deterministic reduction kernels.
 //
@@ -29,8 +31,6 @@
 
 namespace pyiid {
 namespace {
-
-using f4v = float __attribute__((ext_vector_type(4)));
 
 // sin and cos of |rf| <= pi/4: Taylor series, sin through r^9 and cos
 // through r^10 (truncation < 2e-9).
@@ -230,14 +230,14 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 // ---------------------------------------------------------------------------
 template <int M, bool WEIGHTED, bool CONTRACT>
 __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
-    const AtomRec* __restrict__ pos, int n_total, const float* __restrict__ ftab, int n_elem,
-    int nq, double dq, const Item* __restrict__ items, float* __restrict__ gpart,
-    double* __restrict__ fpart, const float* __restrict__ u, float* __restrict__ force_part) {
+    const AtomRec* __restrict__ pos, const int32_t* __restrict__ jperm, int n_total,
+    const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
+    float* __restrict__ gpart, double* __restrict__ fpart, const float* __restrict__ u,
+    float* __restrict__ force_part) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFusedSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
   constexpr int kGS = kGeoJ * 64;  // field stride in geo_f
-  __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kFusedWaves * M : 1];
 
   const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -259,16 +259,6 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
     asm volatile("" : "+v"(qv[m]));
   }
 
-  // Weighted: the pair factor f_i(Q) f_j(Q) is split; the loop sums f_j(Q)
-  // times the pair terms and f_i(Q) multiplies the lane's sums at the end.
-  if constexpr (WEIGHTED) {
-    const int qb = blockIdx.y * kFusedWaves * M;
-    for (int t = threadIdx.x; t < kMaxElem * kFusedWaves * M; t += blockDim.x) {
-      const int e = t / (kFusedWaves * M), qq = t % (kFusedWaves * M);
-      fl[e][qq] = (e < n_elem && qb + qq < nq) ? ftab[(size_t)e * nq + qb + qq] : 0.0f;
-    }
-  }
-
   float aF[M], ax[M], ay[M], az[M];
 #pragma unroll
   for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = 0.0f;
@@ -278,7 +268,7 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
     if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end)
-      tile[threadIdx.x] = pos[jt + threadIdx.x];
+      tile[threadIdx.x] = pos[jperm[jt + threadIdx.x]];
     __syncthreads();
     const int cnt = min(kJTile, j_end - jt);
     for (int js = 0; js < cnt; js += kGeoJ) {
@@ -329,51 +319,23 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
           float w = sn * inv_r;
           float wp = fmaf(sn, c1, -cs * s1) * inv_r;  // sin(Q0 r - dtheta) / r
           float cp = fmaf(cs, c1, sn * s1);           // cos(Q0 r - dtheta)
-          if constexpr (WEIGHTED) {
-            const int ej = static_cast<int>(tile[js + k].w);
-            const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
 #pragma unroll
-            for (int m4 = 0; m4 < M / 4; ++m4) {
-              const f4v fj = fj4[m4];
-#pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                const int m = m4 * 4 + e;
-                const float t = fmaf(qv[m], cs, -w) * fj[e];
-                aF[m] = fmaf(w, fj[e], aF[m]);
-                ax[m] = fmaf(t, dpx, ax[m]);
-                ay[m] = fmaf(t, dpy, ay[m]);
-                az[m] = fmaf(t, dpz, az[m]);
-                float wn = fmaf(k2, w, -wp);
-                float cn = fmaf(k2, cs, -cp);
-                if (e == 3) {
-                  wn = fmaf(lo4, w, wn);
-                  cn = fmaf(lo4, cs, cn);
-                }
-                wp = w;
-                cp = cs;
-                w = wn;
-                cs = cn;
-              }
+          for (int m = 0; m < M; ++m) {
+            const float t = fmaf(qv[m], cs, -w);
+            aF[m] += w;
+            ax[m] = fmaf(t, dpx, ax[m]);
+            ay[m] = fmaf(t, dpy, ay[m]);
+            az[m] = fmaf(t, dpz, az[m]);
+            float wn = fmaf(k2, w, -wp);
+            float cn = fmaf(k2, cs, -cp);
+            if ((m & 3) == 3) {
+              wn = fmaf(lo4, w, wn);
+              cn = fmaf(lo4, cs, cn);
             }
-          } else {
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-              const float t = fmaf(qv[m], cs, -w);
-              aF[m] += w;
-              ax[m] = fmaf(t, dpx, ax[m]);
-              ay[m] = fmaf(t, dpy, ay[m]);
-              az[m] = fmaf(t, dpz, az[m]);
-              float wn = fmaf(k2, w, -wp);
-              float cn = fmaf(k2, cs, -cp);
-              if ((m & 3) == 3) {
-                wn = fmaf(lo4, w, wn);
-                cn = fmaf(lo4, cs, cn);
-              }
-              wp = w;
-              cp = cs;
-              w = wn;
-              cs = cn;
-            }
+            wp = w;
+            cp = cs;
+            w = wn;
+            cs = cn;
           }
         }
       }
@@ -381,11 +343,14 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
     }
   }
 
+  // Weighted: the item's j range is of one element, item.pad, so the pair
+  // factor f_i(Q) f_j(Q) is the same for every term of the lane's sums.
   if constexpr (WEIGHTED) {
     const float* fti = ftab + static_cast<size_t>(pi.w) * nq;
+    const float* ftj = ftab + static_cast<size_t>(it.pad) * nq;
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-      const float f = (q0 + m < nq) ? fti[q0 + m] : 0.0f;
+      const float f = (q0 + m < nq) ? fti[q0 + m] * ftj[q0 + m] : 0.0f;
       aF[m] *= f;
       ax[m] *= f;
       ay[m] *= f;
@@ -429,17 +394,22 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
 // F-only kernel (unordered pairs j < i), Chebyshev recurrence on omega.
 //   grid: x = work item (i tile, j range of the same chain), y = Q group
 //   fpart[item][q] double
+// Item indices are positions in perm.  Weighted: the positions are in
+// element order and the item's j range is of one element, item.pad, so that
+// F = sum_{e,e'} f_e f_e' S_ee' holds per item: the loop sums plain
+// sin(Q r)/r and the lane's sums are weighted by f_i(Q) f_j(Q) at the end.
 // ---------------------------------------------------------------------------
+// 3 waves per SIMD (168 VGPRs) for both variants.
 template <int M, bool WEIGHTED>
-__global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos,
-                                                 const float* __restrict__ ftab, int n_elem,
-                                                 int nq, double dq, const Item* __restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fq_kernel(const AtomRec* __restrict__ pos,
+                                                 const int32_t* __restrict__ perm,
+                                                 const float* __restrict__ ftab, int nq, double dq,
+                                                 const Item* __restrict__ items,
                                                  double* __restrict__ fpart) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFqSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
   constexpr int kGS = kFqGeoJ * 64;
-  __shared__ __attribute__((aligned(16))) float fl[WEIGHTED ? kMaxElem : 1][WEIGHTED ? kWaves * M : 1];
 
   const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
@@ -447,17 +417,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
   const int i = it.i_base + lane;
   const int q0 = (blockIdx.y * kWaves + wave) * M;
   const bool lane_ok = i < it.i_end;
-  const AtomRec pi = pos[lane_ok ? i : it.i_base];
-
-  // Weighted: the loop sums f_j(Q) omega; f_i(Q) multiplies the lane's sums
-  // at the end.
-  if constexpr (WEIGHTED) {
-    const int qb = blockIdx.y * kWaves * M;
-    for (int t = threadIdx.x; t < kMaxElem * kWaves * M; t += blockDim.x) {
-      const int e = t / (kWaves * M), qq = t % (kWaves * M);
-      fl[e][qq] = (e < n_elem && qb + qq < nq) ? ftab[(size_t)e * nq + qb + qq] : 0.0f;
-    }
-  }
+  const AtomRec pi = pos[perm[lane_ok ? i : it.i_base]];
 
   float acc[M];
 #pragma unroll
@@ -467,7 +427,7 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
   const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
   for (int jt = it.jbeg; jt < it.jend; jt += kJTile) {
     __syncthreads();
-    if (jt + (int)threadIdx.x < it.jend) tile[threadIdx.x] = pos[jt + threadIdx.x];
+    if (jt + (int)threadIdx.x < it.jend) tile[threadIdx.x] = pos[perm[jt + threadIdx.x]];
     __syncthreads();
     const int cnt = min(kJTile, it.jend - jt);
     for (int js = 0; js < cnt; js += kFqGeoJ) {
@@ -531,31 +491,13 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
           // odd bins take the sign (-1)^k back.  The choice is wave-uniform.
           if (__all(inv_r == 0.0f || eps >= kChebEpsMin)) {
             float w1 = fmaf(sn, 0.5f * k2hi, cs * s1) * inv_r;
-            if constexpr (WEIGHTED) {
-              const int ej = static_cast<int>(tile[js + k].w);
-              const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
 #pragma unroll
-              for (int m4 = 0; m4 < M / 4; ++m4) {
-                const f4v fj = fj4[m4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                  const int m = m4 * 4 + e;
-                  acc[m] = fmaf(w0, fj[e], acc[m]);
-                  float w2 = fmaf(k2hi, w1, -w0);
-                  if (e == 3) w2 = fmaf(lo4, w1, w2);
-                  w0 = w1;
-                  w1 = w2;
-                }
-              }
-            } else {
-#pragma unroll
-              for (int m = 0; m < M; ++m) {
-                acc[m] += w0;
-                float w2 = fmaf(k2hi, w1, -w0);
-                if ((m & 3) == 3) w2 = fmaf(lo4, w1, w2);
-                w0 = w1;
-                w1 = w2;
-              }
+            for (int m = 0; m < M; ++m) {
+              acc[m] += w0;
+              float w2 = fmaf(k2hi, w1, -w0);
+              if ((m & 3) == 3) w2 = fmaf(lo4, w1, w2);
+              w0 = w1;
+              w1 = w2;
             }
             continue;
           }
@@ -566,31 +508,14 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
           const float sh2 = 0.25f * eps;
           float dd = fmaf(cs, sgn * s1, -2.0f * sn * sh2) * inv_r;
           float y = w0;
-          if constexpr (WEIGHTED) {
-            const int ej = static_cast<int>(tile[js + k].w);
-            const f4v* fj4 = reinterpret_cast<const f4v*>(&fl[ej][wave * M]);
 #pragma unroll
-            for (int m4 = 0; m4 < M / 4; ++m4) {
-              const f4v fj = fj4[m4];
-#pragma unroll
-              for (int e = 0; e < 4; ++e) {
-                const int m = m4 * 4 + e;
-                const float ff = (e & 1) ? fj[e] * sgn : fj[e];
-                acc[m] = fmaf(y, ff, acc[m]);
-                y += dd;
-                dd = fmaf(-eps, y, dd);
-              }
-            }
-          } else {
-#pragma unroll
-            for (int m = 0; m < M; m += 2) {
-              acc[m] += y;
-              y += dd;
-              dd = fmaf(-eps, y, dd);
-              acc[m + 1] = fmaf(sgn, y, acc[m + 1]);
-              y += dd;
-              dd = fmaf(-eps, y, dd);
-            }
+          for (int m = 0; m < M; m += 2) {
+            acc[m] += y;
+            y += dd;
+            dd = fmaf(-eps, y, dd);
+            acc[m + 1] = fmaf(sgn, y, acc[m + 1]);
+            y += dd;
+            dd = fmaf(-eps, y, dd);
           }
         }
       }
@@ -600,8 +525,9 @@ __global__ __launch_bounds__(256) void fq_kernel(const AtomRec* __restrict__ pos
 
   if constexpr (WEIGHTED) {
     const float* fti = ftab + static_cast<size_t>(pi.w) * nq;
+    const float* ftj = ftab + static_cast<size_t>(it.pad) * nq;
 #pragma unroll
-    for (int m = 0; m < M; ++m) acc[m] *= (q0 + m < nq) ? fti[q0 + m] : 0.0f;
+    for (int m = 0; m < M; ++m) acc[m] *= (q0 + m < nq) ? fti[q0 + m] * ftj[q0 + m] : 0.0f;
   }
   const double mine = lane_reduce_f64<M>(acc, smem, wave, lane);
   if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine;
@@ -900,8 +826,8 @@ int grid_for(int64_t total, int block) {
 }  // namespace
 
 template <int M>
-hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
-                          int n_elem, int nq, double dq, const Item* items, int n_items,
+hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, const int32_t* jperm, int n_total,
+                          const float* ftab, int nq, double dq, const Item* items, int n_items,
                           bool weighted, float* gpart, double* fpart, const float* u,
                           float* force_part) {
   dim3 grid(n_items, (nq + kFusedWaves * M - 1) / (kFusedWaves * M));
@@ -909,32 +835,32 @@ hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, int n_total, const
   const bool contract = (u != nullptr);
   if (weighted) {
     if (contract)
-      hipLaunchKernelGGL((fused_grad_kernel<M, true, true>), grid, block, 0, st, pos, n_total,
-                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, true, true>), grid, block, 0, st, pos, jperm,
+                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
     else
-      hipLaunchKernelGGL((fused_grad_kernel<M, true, false>), grid, block, 0, st, pos, n_total,
-                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, true, false>), grid, block, 0, st, pos, jperm,
+                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
   } else {
     if (contract)
-      hipLaunchKernelGGL((fused_grad_kernel<M, false, true>), grid, block, 0, st, pos, n_total,
-                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, false, true>), grid, block, 0, st, pos, jperm,
+                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
     else
-      hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, n_total,
-                         ftab, n_elem, nq, dq, items, gpart, fpart, u, force_part);
+      hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, jperm,
+                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, int n_total,
-                             const float* ftab, int n_elem, int nq, double dq, const Item* items,
+hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
+                             int n_total, const float* ftab, int nq, double dq, const Item* items,
                              int n_items, bool weighted, float* gpart, double* fpart,
                              const float* u, float* force_part) {
   if (n_items <= 0) return hipSuccess;
   if (m == 16)
-    return launch_fused_m<16>(st, pos, n_total, ftab, n_elem, nq, dq, items, n_items, weighted,
+    return launch_fused_m<16>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items, weighted,
                               gpart, fpart, u, force_part);
   if (m == 32)
-    return launch_fused_m<32>(st, pos, n_total, ftab, n_elem, nq, dq, items, n_items, weighted,
+    return launch_fused_m<32>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items, weighted,
                               gpart, fpart, u, force_part);
   return hipErrorInvalidValue;
 }
@@ -975,18 +901,17 @@ hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double
   return hipGetLastError();
 }
 
-hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
-                     int n_elem, int nq, double dq, const Item* items, int n_items,
-                     bool weighted, double* fpart) {
+hipError_t launch_fq(hipStream_t st, const AtomRec* pos, const int32_t* perm, const float* ftab,
+                     int nq, double dq, const Item* items, int n_items, bool weighted,
+                     double* fpart) {
   constexpr int M = kFqM;
-  (void)n_total;
   if (n_items <= 0) return hipSuccess;
   dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
   if (weighted)
-    hipLaunchKernelGGL((fq_kernel<M, true>), grid, dim3(256), 0, st, pos, ftab, n_elem, nq, dq,
+    hipLaunchKernelGGL((fq_kernel<M, true>), grid, dim3(256), 0, st, pos, perm, ftab, nq, dq,
                        items, fpart);
   else
-    hipLaunchKernelGGL((fq_kernel<M, false>), grid, dim3(256), 0, st, pos, ftab, n_elem, nq, dq,
+    hipLaunchKernelGGL((fq_kernel<M, false>), grid, dim3(256), 0, st, pos, perm, ftab, nq, dq,
                        items, fpart);
   return hipGetLastError();
 }

@@ -20,7 +20,6 @@ constexpr int kFqM = 64;
 constexpr int kWaves = 4;      // waves per block in the F-only and contracted-force kernels
 constexpr int kFusedWaves = 8; // waves per block in the fused F + gradient kernel
 constexpr int kJTile = 256;    // atoms staged in LDS per j tile
-constexpr int kMaxElem = 8;    // element-table rows staged in LDS (weighted kernels)
 
 // Work item of the Debye kernels.  Structures ("chains") are stored back to
 // back; an item is a 64-atom i tile [i_base, i_end) of one chain against the
@@ -41,8 +40,10 @@ struct Range {
 // gpart[item.split][i][c][q] (float) for i in the item's tile.  Contract
 // (u != null): force_part[item][qchunk][64][3] = sum_q u[chain][q] * grad(i, c, q),
 // qchunk over ceil(nq / (kFusedWaves m)) * kFusedWaves.  fpart[item][q]: F partial (double).
-hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, int n_total,
-                             const float* ftab, int n_elem, int nq, double dq, const Item* items,
+// The j indices of an item are positions in jperm; weighted: item.pad is the
+// element of the item's j range.
+hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
+                             int n_total, const float* ftab, int nq, double dq, const Item* items,
                              int n_items, bool weighted, float* gpart, double* fpart,
                              const float* u, float* force_part);
 // Contracted forces over unordered pairs: items are 64-atom tile pairs (I, J),
@@ -60,10 +61,11 @@ hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int
                                     const Range* tiles, int n_tiles, double conv, double* out);
 // uq[b][q] = u[b][q] * float32(q dq)
 hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
-// Unordered-pair F only: fpart[item][q] (double).
-hipError_t launch_fq(hipStream_t st, const AtomRec* pos, int n_total, const float* ftab,
-                     int n_elem, int nq, double dq, const Item* items, int n_items,
-                     bool weighted, double* fpart);
+// Unordered-pair F only: fpart[item][q] (double).  Item indices are
+// positions in perm; weighted: item.pad is the element of the item's j range.
+hipError_t launch_fq(hipStream_t st, const AtomRec* pos, const int32_t* perm, const float* ftab,
+                     int nq, double dq, const Item* items, int n_items, bool weighted,
+                     double* fpart);
 // out[r][q] = scale[r][q] * sum_{p in ranges[r]} part[p][q], r < n_rows (FP64)
 hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range* ranges,
                                 int n_rows, int nq, const double* scale, double* out_f64,

@@ -163,6 +163,47 @@ def test_contracted_forces_both_kernels(ctx, qbin, nq):
     assert rel(f, f_want) < TOL
 
 
+def test_ten_elements_and_mixed_batch(ctx):
+    """All ten tabulated elements in one particle, in random order: F, the
+    FP64 gradient and the F(Q)-Rw forces (10 rows x 1,700 bins exceed the
+    unordered-pair force kernel's LDS, so the fused kernel's contracted
+    variant runs), then a batch of chains of different compositions (one
+    element, two, ten, a single atom) through the F-only kernel, each chain
+    against the oracle."""
+    zs = np.array([6, 8, 14, 26, 28, 29, 46, 47, 78, 79])
+    rs = np.random.RandomState(11)
+    base = structures.fcc_nearest(201, a=3.9)
+    pos = structures.jitter(base, 0.05, seed=12).astype(np.float32)
+    numbers = zs[rs.randint(0, len(zs), len(pos))]
+    qbin, nq, q_first = 0.02, 1700, 30
+    per_atom = load(ctx, pos, numbers, qbin, nq)
+    f_want = oracle.fq(pos, per_atom, qbin)
+    assert rel(ctx.fq(_lib.SLOT_FQ, qbin, nq), f_want) < TOL
+    g_want = oracle.grad_fq(pos, per_atom, qbin)
+    assert rel(ctx.grad_fq(_lib.SLOT_FQ, qbin, len(pos), nq, f64=True), g_want) < TOL
+    target = oracle.fq((base * np.float32(1.002)).astype(np.float32), per_atom, qbin)[q_first:]
+    e, f = ctx.fq_energy_forces(_lib.SLOT_FQ, qbin, q_first, target, _lib.POT_RW, 10.0, len(pos))
+    assert abs(e - 10.0 * oracle.get_rw(target, f_want[q_first:])[0]) < 1e-5
+    assert rel(f, 10.0 * oracle.wrap_grad_rw(g_want[:, :, q_first:], f_want[q_first:], target)) < TOL
+
+    chains = [structures.jitter(structures.fcc_nearest(n, a=3.9), 0.05, seed=20 + n).astype(np.float32)
+              for n in (30, 41, 57, 1)]
+    zc = [np.full(30, 79), structures.random_alloy(41, 78, 28, seed=3),
+          zs[rs.randint(0, len(zs), 57)], np.array([29])]
+    allz = np.concatenate(zc)
+    table, index, elems = element_table(allz, qbin, nq)
+    ctx.set_batch(chains)
+    ctx.set_elements(index)
+    ctx.set_scatter(_lib.SLOT_FQ, table)
+    fq = ctx.fq(_lib.SLOT_FQ, qbin, nq, n_chain=len(chains))
+    for b, (p, z) in enumerate(zip(chains, zc)):
+        want = oracle.fq(p, table[np.searchsorted(elems, z)], qbin)
+        if len(p) < 2:
+            assert np.all(fq[b] == 0.0)
+        else:
+            assert rel(fq[b], want) < TOL
+
+
 def test_device_pointer_mode_matches_host_mode(ctx):
     """PYIID_DEVICE (inputs and outputs as device pointers, ordered on the
     context stream; what bench.py and torch callers use) gives the same
