@@ -227,6 +227,9 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 //   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
 //   CONTRACT: instead of gpart, force_part[item][qchunk][lane][c] =
 //             sum_q u[chain][q] * gpart(i, c, q)
+// Weighted: the item's j indices are positions in jperm (the atoms in
+// element order) and its j range is of one element, item.pad; the loop is
+// the single-element one and f_i(Q) f_j(Q) scales the lane's sums at the end.
 // ---------------------------------------------------------------------------
 template <int M, bool WEIGHTED, bool CONTRACT>
 __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
     if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end)
-      tile[threadIdx.x] = pos[jperm[jt + threadIdx.x]];
+      tile[threadIdx.x] = pos[WEIGHTED ? jperm[jt + threadIdx.x] : jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, j_end - jt);
     for (int js = 0; js < cnt; js += kGeoJ) {
@@ -394,10 +397,11 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
 // F-only kernel (unordered pairs j < i), Chebyshev recurrence on omega.
 //   grid: x = work item (i tile, j range of the same chain), y = Q group
 //   fpart[item][q] double
-// Item indices are positions in perm.  Weighted: the positions are in
-// element order and the item's j range is of one element, item.pad, so that
+// Weighted: item indices are positions in perm, the atoms in element order,
+// and the item's j range is of one element, item.pad, so that
 // F = sum_{e,e'} f_e f_e' S_ee' holds per item: the loop sums plain
 // sin(Q r)/r and the lane's sums are weighted by f_i(Q) f_j(Q) at the end.
+// Unweighted: item indices are atom indices (perm is not read).
 // ---------------------------------------------------------------------------
 // 3 waves per SIMD (168 VGPRs) for both variants.
 template <int M, bool WEIGHTED>
@@ -417,7 +421,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   const int i = it.i_base + lane;
   const int q0 = (blockIdx.y * kWaves + wave) * M;
   const bool lane_ok = i < it.i_end;
-  const AtomRec pi = pos[perm[lane_ok ? i : it.i_base]];
+  const int ii = lane_ok ? i : it.i_base;
+  const AtomRec pi = pos[WEIGHTED ? perm[ii] : ii];
 
   float acc[M];
 #pragma unroll
@@ -427,7 +432,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
   for (int jt = it.jbeg; jt < it.jend; jt += kJTile) {
     __syncthreads();
-    if (jt + (int)threadIdx.x < it.jend) tile[threadIdx.x] = pos[perm[jt + threadIdx.x]];
+    if (jt + (int)threadIdx.x < it.jend)
+      tile[threadIdx.x] = pos[WEIGHTED ? perm[jt + threadIdx.x] : jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, it.jend - jt);
     for (int js = 0; js < cnt; js += kFqGeoJ) {

@@ -40,8 +40,8 @@ struct Range {
 // gpart[item.split][i][c][q] (float) for i in the item's tile.  Contract
 // (u != null): force_part[item][qchunk][64][3] = sum_q u[chain][q] * grad(i, c, q),
 // qchunk over ceil(nq / (kFusedWaves m)) * kFusedWaves.  fpart[item][q]: F partial (double).
-// The j indices of an item are positions in jperm; weighted: item.pad is the
-// element of the item's j range.
+// Weighted: the j indices of an item are positions in jperm and item.pad is
+// the element of the item's j range; unweighted: atom indices.
 hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
                              int n_total, const float* ftab, int nq, double dq, const Item* items,
                              int n_items, bool weighted, float* gpart, double* fpart,
@@ -61,8 +61,9 @@ hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int
                                     const Range* tiles, int n_tiles, double conv, double* out);
 // uq[b][q] = u[b][q] * float32(q dq)
 hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
-// Unordered-pair F only: fpart[item][q] (double).  Item indices are
-// positions in perm; weighted: item.pad is the element of the item's j range.
+// Unordered-pair F only: fpart[item][q] (double).  Weighted: item indices
+// are positions in perm and item.pad is the element of the item's j range;
+// unweighted: atom indices.
 hipError_t launch_fq(hipStream_t st, const AtomRec* pos, const int32_t* perm, const float* ftab,
                      int nq, double dq, const Item* items, int n_items, bool weighted,
                      double* fpart);
