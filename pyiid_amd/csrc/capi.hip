@@ -173,6 +173,7 @@ struct pyiid_ctx {
 
   // launch shapes (set by the autotuner per problem shape)
   int grad_m = 32;             // Q bins per wave of the fused kernel
+  int fq_m = kFqM;             // Q bins per wave of the F-only kernel
   int64_t fused_waves = 32768; // target wave count that sets the fused j split
   int64_t pf_waves = 8192;     // target wave count that sets the force Q groups
   bool autotune = true;
@@ -318,12 +319,13 @@ int upload_items(pyiid_ctx*, ItemSet& s, const std::vector<Item>& v, const std::
   return PYIID_OK;
 }
 
-// Unordered-pair items: per chain, per 64-atom i tile, j ranges of the chain
-// below the tile's end.  Ranges: one per chain.  Item indices are positions
-// in perm, the chain's atoms in element order (stable) for a weighted table
-// and in their own order otherwise; for a weighted table no j range crosses
-// an element boundary and item.pad is the range's element, so that the
-// kernel sums plain sin(Q r)/r and weights the sums by f_i(Q) f_j(Q) once.
+// Unordered-pair items: per chain, per i tile of at most 64 atoms, j ranges
+// of the chain below the tile's end.  Ranges: one per chain.  Item indices
+// are positions in perm, the chain's atoms in element order (stable) for a
+// weighted table and in their own order otherwise; for a weighted table no i
+// tile or j range crosses an element boundary, item.split is the i tile's
+// element and item.pad the j range's, so that the kernel sums plain
+// sin(Q r)/r and weights each bin's total by f_i(Q) f_j(Q) once.
 int ensure_fq_items(pyiid_ctx* c, bool weighted) {
   ItemSet& s = c->fq_items;
   const int key = weighted ? 1 : 0;
@@ -347,8 +349,10 @@ int ensure_fq_items(pyiid_ctx* c, bool weighted) {
       for (int64_t p = o + nb - 2; p >= o; --p)
         seg_end[p - o] = el(perm[p]) == el(perm[p + 1]) ? seg_end[p + 1 - o] : p + 1;
     }
-    for (int64_t ib = o; ib < o + nb; ib += 64, ++tile) {
-      const int64_t ie = std::min(o + nb, ib + 64);
+    for (int64_t ib = o, ie = o; ib < o + nb; ib = ie, ++tile) {
+      ie = std::min(o + nb, ib + 64);
+      if (weighted) ie = std::min(ie, seg_end[ib - o]);
+      const int ei = weighted ? el(perm[ib]) : 0;
       for (int64_t jb = o; jb < ie;) {
         int64_t je = std::min(ie, jb + jc);
         int ej = 0;
@@ -357,7 +361,7 @@ int ensure_fq_items(pyiid_ctx* c, bool weighted) {
           ej = el(perm[jb]);
         }
         v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(jb),
-                         static_cast<int>(je), 0, b, tile, ej});
+                         static_cast<int>(je), ei, b, tile, ej});
         jb = je;
       }
     }
@@ -514,6 +518,15 @@ int64_t min_chain_atoms(pyiid_ctx* c) {
   return c->n_chain ? m : 0;
 }
 
+int tune_fq(pyiid_ctx* c, int slot, double qbin);
+
+hipError_t launch_fq_items(pyiid_ctx* c, const Table& t, double qbin) {
+  return launch_fq(c->stream, c->fq_m, c->pos.as<AtomRec>(), c->fq_items.perm.as<int32_t>(),
+                   t.tab.as<float>(), t.nq, static_cast<double>(static_cast<float>(qbin)),
+                   c->fq_items.items.as<Item>(), c->fq_items.n_items, weighted_table(t),
+                   c->fpart.as<double>());
+}
+
 // F(Q) of every chain (full grid) into c->fq64 [B][nq] (double) and c->fq32.
 int compute_fq(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
@@ -524,11 +537,9 @@ int compute_fq(pyiid_ctx* c, int slot, double qbin) {
   CHK(ensure_norm(c, slot));
   CHK(ensure_fq_items(c, weighted_table(t)));
   CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, c->fq_items.n_items) * nq));
-  const double dq = static_cast<double>(static_cast<float>(qbin));
+  CHK(tune_fq(c, slot, qbin));
   Timed tm(c, 1);
-  HIPCHK(launch_fq(c->stream, c->pos.as<AtomRec>(), c->fq_items.perm.as<int32_t>(),
-                   t.tab.as<float>(), nq, dq, c->fq_items.items.as<Item>(),
-                   c->fq_items.n_items, weighted_table(t), c->fpart.as<double>()));
+  HIPCHK(launch_fq_items(c, t, qbin));
   CHK(tm.done());
   // chains without pairs have empty ranges: their F is 0
   HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), c->fq_items.ranges.as<Range>(), B,
@@ -694,6 +705,40 @@ int tune_contract(pyiid_ctx* c, int slot, double qbin) {
   }
   c->pf_waves = best_w;
   c->tuned[key] = {best_w};
+  return PYIID_OK;
+}
+
+// F-only kernel: Q bins per wave.  A block spans 4 waves of m bins and every
+// span repeats the per-pair setup, so the best m depends on nq (how full the
+// last span is) and on the item shapes.
+int tune_fq(pyiid_ctx* c, int slot, double qbin) {
+  Table& t = c->tab[slot];
+  const std::vector<int64_t> key{2, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
+                                 static_cast<int64_t>(c->struct_stamp)};
+  auto hit = c->tuned.find(key);
+  if (hit != c->tuned.end()) {
+    c->fq_m = static_cast<int>(hit->second[0]);
+    return PYIID_OK;
+  }
+  c->fq_m = kFqM;
+  if (c->capturing || !c->autotune) return PYIID_OK;
+  double best = 1e300;
+  int best_m = kFqM;
+  for (int m : {64, 80, 96, 112}) {
+    c->fq_m = m;
+    double ms = 0.0;
+    HIPCHK(launch_fq_items(c, t, qbin));
+    CHK(time_launch(c, [&]() {
+      HIPCHK(launch_fq_items(c, t, qbin));
+      return PYIID_OK;
+    }, &ms));
+    if (ms < best) {
+      best = ms;
+      best_m = m;
+    }
+  }
+  c->fq_m = best_m;
+  c->tuned[key] = {best_m};
   return PYIID_OK;
 }
 

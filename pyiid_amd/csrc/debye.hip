@@ -162,8 +162,8 @@ constexpr int kFusedSmemBytes = cmax3(kFusedWaves * kRedM * kRedRow * 4,
                                       kFusedWaves * 64 * kStoreRow * 4, kJTile * 32 + kGeoBytes);
 // F-only kernel: rounds of kFqGeoJ j atoms share the per-pair setup (FP32,
 // [field][k][lane]): 1/r, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, eps of d,
-// (-1)^k (dtheta = k pi + d), and the anchor sin, cos(Q0 r) of every wave's Q
-// chunk.  Rounds of 8 keep the block at 37 KB of LDS.
+// (-1)^k (dtheta = k pi + d), and the anchor sin, cos(Q0 r) of each of the
+// block's kWaves Q chunks.  Rounds of 8 keep the block at 36 KB of LDS.
 constexpr int kFqGeoJ = 8;
 constexpr int kFqGeoFields = 6 + 2 * kWaves;
 constexpr int kFqGeoBytes = kFqGeoJ * 64 * kFqGeoFields * 4;
@@ -171,20 +171,24 @@ constexpr int kFqSmemBytes = kWaves * kRedM * kRedRow * 4 > kJTile * 32 + kFqGeo
                                  ? kWaves * kRedM * kRedRow * 4
                                  : kJTile * 32 + kFqGeoBytes;
 
-// Sum acc[m] over the wave's 64 lanes in FP64 (through LDS); returns, in lane
-// L < M, the total of bin L.  Must be reached by every wave of the block.
-template <int M>
+// Sum acc[OFF + n], n < N <= 64, over the wave's 64 lanes in FP64 (through
+// LDS); returns, in lane L < N, the total of bin OFF + L.  Must be reached by
+// every wave of the block.
+template <int M, int OFF = 0, int N = M>
 __device__ __forceinline__ double lane_reduce_f64(const float (&acc)[M], unsigned char* smem,
                                                   int wave, int lane) {
+  static_assert(N % 8 == 0 && N <= 64 && OFF + N <= M, "bins per pass");
   float* red = reinterpret_cast<float*>(smem) + wave * kRedM * kRedRow;
   double mine = 0.0;
 #pragma unroll
-  for (int m0 = 0; m0 < M; m0 += kRedM) {
+  for (int m0 = 0; m0 < N; m0 += kRedM) {
+    constexpr int kLast = M - 1;
     __syncthreads();
 #pragma unroll
-    for (int mm = 0; mm < kRedM; ++mm) red[mm * kRedRow + lane] = acc[m0 + mm];
+    for (int mm = 0; mm < kRedM; ++mm)
+      if (m0 + mm < N) red[mm * kRedRow + lane] = acc[min(OFF + m0 + mm, kLast)];
     __syncthreads();
-    if (lane >= m0 && lane < m0 + kRedM) {
+    if (lane >= m0 && lane < min(m0 + kRedM, N)) {
       const float* row = red + (lane - m0) * kRedRow;
       double t = 0.0;
       for (int l = 0; l < 64; ++l) t += static_cast<double>(row[l]);
@@ -193,7 +197,6 @@ __device__ __forceinline__ double lane_reduce_f64(const float (&acc)[M], unsigne
   }
   return mine;
 }
-
 
 // Coalesced store of a wave's 64 (atoms) x M (bins) tile: lane l holds row l.
 // Rows go through LDS and leave as contiguous M-float row segments.
@@ -397,19 +400,96 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
 // F-only kernel (unordered pairs j < i), Chebyshev recurrence on omega.
 //   grid: x = work item (i tile, j range of the same chain), y = Q group
 //   fpart[item][q] double
-// Weighted: item indices are positions in perm, the atoms in element order,
-// and the item's j range is of one element, item.pad, so that
-// F = sum_{e,e'} f_e f_e' S_ee' holds per item: the loop sums plain
-// sin(Q r)/r and the lane's sums are weighted by f_i(Q) f_j(Q) at the end.
+// Weighted: item indices are positions in perm, the atoms in element order;
+// the item's i tile is of one element, item.split, and its j range of one
+// element, item.pad, so that F = sum_{e,e'} f_e f_e' S_ee' holds per item:
+// the loop sums plain sin(Q r)/r and each bin's total is weighted by
+// f_i(Q) f_j(Q) at the end.
 // Unweighted: item indices are atom indices (perm is not read).
 // ---------------------------------------------------------------------------
-// 3 waves per SIMD (168 VGPRs) for both variants.
+// F-only inner loops over NP j atoms of a round at once (their LDS offsets
+// o[p]), adding sin(Q_m r)/r of each pair to acc[m] for the wave's M bins.
+// Three-term form: w_{m+2} = 2 cos(dtheta) w_{m+1} - w_m with 2 cos(dtheta)
+// as hi + lo, the low part added every fourth bin as 4 lo.
+template <int M, int NP>
+__device__ __forceinline__ void fq_three_term(float (&acc)[M], const float* geo_f, int kGS,
+                                              const int (&o)[NP], int chunk) {
+  float w0[NP], w1[NP], k2[NP], lo4[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const float inv_r = geo_f[o[p]], s1 = geo_f[o[p] + kGS];
+    k2[p] = geo_f[o[p] + 2 * kGS];
+    lo4[p] = geo_f[o[p] + 3 * kGS];
+    const float sn = geo_f[o[p] + (6 + 2 * chunk) * kGS], cs = geo_f[o[p] + (7 + 2 * chunk) * kGS];
+    w0[p] = sn * inv_r;
+    w1[p] = fmaf(sn, 0.5f * k2[p], cs * s1) * inv_r;
+  }
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    float t = w0[0];
+#pragma unroll
+    for (int p = 1; p < NP; ++p) t += w0[p];
+    acc[m] += t;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      float w2 = fmaf(k2[p], w1[p], -w0[p]);
+      if ((m & 3) == 3) w2 = fmaf(lo4[p], w1[p], w2);
+      w0[p] = w1[p];
+      w1[p] = w2;
+    }
+  }
+}
+
+// Reinsch difference form on y_m = (-1)^(k m) w_m = sin(Q0 r + m d) / r
+// (dtheta = k pi + d):  y_{m+1} = y_m + D_{m+1},  D_{m+2} = D_{m+1} - eps y_{m+1},
+// eps = 4 sin^2(d/2); odd bins take the sign (-1)^k back.
+template <int M, int NP>
+__device__ __forceinline__ void fq_reinsch(float (&acc)[M], const float* geo_f, int kGS,
+                                           const int (&o)[NP], int chunk) {
+  float y[NP], dd[NP], eps[NP], sgn[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const float inv_r = geo_f[o[p]], s1 = geo_f[o[p] + kGS];
+    eps[p] = geo_f[o[p] + 4 * kGS];
+    sgn[p] = geo_f[o[p] + 5 * kGS];
+    const float sn = geo_f[o[p] + (6 + 2 * chunk) * kGS], cs = geo_f[o[p] + (7 + 2 * chunk) * kGS];
+    // D_1 = y_1 - y_0 = (sin(t + d) - sin t) / r
+    //     = 2 sin(d/2) (cos t cos(d/2) - sin t sin(d/2)) / r,
+    // from sin(d/2)^2 = eps / 4 and sin(d/2) cos(d/2) = sgn s1 / 2
+    // (no cancellation for small d).
+    dd[p] = fmaf(cs, sgn[p] * s1, -2.0f * sn * (0.25f * eps[p])) * inv_r;
+    y[p] = sn * inv_r;
+  }
+#pragma unroll
+  for (int m = 0; m < M; m += 2) {
+    float t = y[0];
+#pragma unroll
+    for (int p = 1; p < NP; ++p) t += y[p];
+    acc[m] += t;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      y[p] += dd[p];
+      dd[p] = fmaf(-eps[p], y[p], dd[p]);
+    }
+    float u = acc[m + 1];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) u = fmaf(sgn[p], y[p], u);
+    acc[m + 1] = u;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      y[p] += dd[p];
+      dd[p] = fmaf(-eps[p], y[p], dd[p]);
+    }
+  }
+}
+
+// M bins per wave (a multiple of 8, at most 128); at least 3 waves per SIMD.
 template <int M, bool WEIGHTED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fq_kernel(const AtomRec* __restrict__ pos,
-                                                 const int32_t* __restrict__ perm,
-                                                 const float* __restrict__ ftab, int nq, double dq,
-                                                 const Item* __restrict__ items,
-                                                 double* __restrict__ fpart) {
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(3))) void
+fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
+          const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
+          double* __restrict__ fpart) {
+  static_assert(M % 8 == 0 && M <= 128, "bins per wave");
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFqSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
@@ -419,7 +499,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = it.i_base + lane;
-  const int q0 = (blockIdx.y * kWaves + wave) * M;
+  // The block's kWaves Q chunks go to its waves in an order rotated by a hash
+  // of the block index: in a Q group with fewer chunks than waves (the last,
+  // when kWaves M does not divide nq) the working waves of the blocks that
+  // share a CU then fall on different SIMDs (wave w of a block runs on SIMD w).
+  const int chunk = (wave + static_cast<int>((blockIdx.x * 0x9E3779B1u) >> 30)) & (kWaves - 1);
+  const int q0 = (blockIdx.y * kWaves + chunk) * M;
   const bool lane_ok = i < it.i_end;
   const int ii = lane_ok ? i : it.i_base;
   const AtomRec pi = pos[WEIGHTED ? perm[ii] : ii];
@@ -432,7 +517,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
   const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
   for (int jt = it.jbeg; jt < it.jend; jt += kJTile) {
     __syncthreads();
-    if (jt + (int)threadIdx.x < it.jend)
+    if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < it.jend)
       tile[threadIdx.x] = pos[WEIGHTED ? perm[jt + threadIdx.x] : jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, it.jend - jt);
@@ -478,65 +563,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
       }
       __syncthreads();
       if (q0 < nq) {  // wave-uniform
+        // When every kept pair of the wave has |d| >= 0.2 the three-term form
+        // with the two-float coefficient (exact frequency on average,
+        // rounding amplified by at most 1/sin|d| < 5) is accurate enough;
+        // otherwise the wave runs the Reinsch form, whose rounding stays O(1)
+        // in the step count.  The choice is wave-uniform: bit k of `three`
+        // marks the round's j atoms of the three-term form, and each form
+        // runs as a loop of its own over its atoms (one loop with a branch per
+        // atom makes the compiler route the sums through copies).
+        uint32_t three = 0;
         for (int k = 0; k < kk; ++k) {
           const int o = k * 64 + lane;
-          const float inv_r = geo_f[o], s1 = geo_f[o + kGS], k2hi = geo_f[o + 2 * kGS],
-                      lo4 = geo_f[o + 3 * kGS], eps = geo_f[o + 4 * kGS],
-                      sgn = geo_f[o + 5 * kGS];
-          const float sn = geo_f[o + (6 + 2 * wave) * kGS], cs = geo_f[o + (7 + 2 * wave) * kGS];
-          float w0 = sn * inv_r;
-          // When every kept pair of the wave has |d| >= 0.2 the
-          // three-term form w2 = 2 cos(dtheta) w1 - w0 with the two-float
-          // coefficient (one FMA per bin, the low part added every fourth bin:
-          // exact frequency on average, rounding amplified by at most
-          // 1/sin|d| < 5) is accurate enough.
-          // Otherwise the wave runs the Reinsch difference form on the
-          // sequence y_m = (-1)^(k m) w_m = sin(Q0 r + m d) / r:
-          //   y_{m+1} = y_m + D_{m+1},  D_{m+2} = D_{m+1} - eps y_{m+1},
-          // eps = 4 sin^2(d/2), whose rounding stays O(1) in the step count;
-          // odd bins take the sign (-1)^k back.  The choice is wave-uniform.
-          if (__all(inv_r == 0.0f || eps >= kChebEpsMin)) {
-            float w1 = fmaf(sn, 0.5f * k2hi, cs * s1) * inv_r;
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-              acc[m] += w0;
-              float w2 = fmaf(k2hi, w1, -w0);
-              if ((m & 3) == 3) w2 = fmaf(lo4, w1, w2);
-              w0 = w1;
-              w1 = w2;
-            }
-            continue;
-          }
-          // D_1 = y_1 - y_0 = (sin(t + d) - sin t) / r
-          //     = 2 sin(d/2) (cos t cos(d/2) - sin t sin(d/2)) / r,
-          // from sin(d/2)^2 = eps / 4 and sin(d/2) cos(d/2) = sgn s1 / 2
-          // (no cancellation for small d).
-          const float sh2 = 0.25f * eps;
-          float dd = fmaf(cs, sgn * s1, -2.0f * sn * sh2) * inv_r;
-          float y = w0;
-#pragma unroll
-          for (int m = 0; m < M; m += 2) {
-            acc[m] += y;
-            y += dd;
-            dd = fmaf(-eps, y, dd);
-            acc[m + 1] = fmaf(sgn, y, acc[m + 1]);
-            y += dd;
-            dd = fmaf(-eps, y, dd);
-          }
+          if (__all(geo_f[o] == 0.0f || geo_f[o + 4 * kGS] >= kChebEpsMin)) three |= 1u << k;
         }
+        // Two j atoms at a time: two independent recurrences per bin keep the
+        // wave issuing while one waits on its previous step.  An odd atom
+        // left over runs alone after the loop.
+        uint32_t left = three;
+        for (; __builtin_popcount(left) >= 2; left &= left - 1, left &= left - 1) {
+          const int ka = __builtin_ctz(left), kb = __builtin_ctz(left & (left - 1));
+          fq_three_term<M, 2>(acc, geo_f, kGS, {ka * 64 + lane, kb * 64 + lane}, chunk);
+        }
+        if (left != 0) fq_three_term<M, 1>(acc, geo_f, kGS, {__builtin_ctz(left) * 64 + lane}, chunk);
+        left = ~three & ((1u << kk) - 1);
+        for (; __builtin_popcount(left) >= 2; left &= left - 1, left &= left - 1) {
+          const int ka = __builtin_ctz(left), kb = __builtin_ctz(left & (left - 1));
+          fq_reinsch<M, 2>(acc, geo_f, kGS, {ka * 64 + lane, kb * 64 + lane}, chunk);
+        }
+        if (left != 0) fq_reinsch<M, 1>(acc, geo_f, kGS, {__builtin_ctz(left) * 64 + lane}, chunk);
       }
       __syncthreads();
     }
   }
 
-  if constexpr (WEIGHTED) {
-    const float* fti = ftab + static_cast<size_t>(pi.w) * nq;
-    const float* ftj = ftab + static_cast<size_t>(it.pad) * nq;
-#pragma unroll
-    for (int m = 0; m < M; ++m) acc[m] *= (q0 + m < nq) ? fti[q0 + m] * ftj[q0 + m] : 0.0f;
+  // lane l keeps bin q0 + l, and bin q0 + 64 + l when M > 64; weighted: the
+  // item's atoms are of one element on each side (item.split for i, item.pad
+  // for j), so each bin's total takes f_i(Q) f_j(Q) once
+  auto weight = [&](int q) -> double {
+    if constexpr (WEIGHTED)
+      return static_cast<double>(ftab[static_cast<size_t>(it.split) * nq + q] *
+                                 ftab[static_cast<size_t>(it.pad) * nq + q]);
+    return 1.0;
+  };
+  constexpr int kLo = M < 64 ? M : 64;
+  const double mine = lane_reduce_f64<M, 0, kLo>(acc, smem, wave, lane);
+  if (lane < kLo && q0 + lane < nq)
+    fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine * weight(q0 + lane);
+  if constexpr (M > 64) {
+    const double hi = lane_reduce_f64<M, 64, M - 64>(acc, smem, wave, lane);
+    if (lane < M - 64 && q0 + 64 + lane < nq)
+      fpart[(size_t)blockIdx.x * nq + q0 + 64 + lane] = hi * weight(q0 + 64 + lane);
   }
-  const double mine = lane_reduce_f64<M>(acc, smem, wave, lane);
-  if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = mine;
 }
 
 // ---------------------------------------------------------------------------
@@ -907,18 +984,42 @@ hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double
   return hipGetLastError();
 }
 
-hipError_t launch_fq(hipStream_t st, const AtomRec* pos, const int32_t* perm, const float* ftab,
-                     int nq, double dq, const Item* items, int n_items, bool weighted,
-                     double* fpart) {
-  constexpr int M = kFqM;
-  if (n_items <= 0) return hipSuccess;
+template <int M, bool WEIGHTED>
+static void launch_fq_m(hipStream_t st, const AtomRec* pos, const int32_t* perm,
+                        const float* ftab, int nq, double dq, const Item* items, int n_items,
+                        double* fpart) {
   dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
-  if (weighted)
-    hipLaunchKernelGGL((fq_kernel<M, true>), grid, dim3(256), 0, st, pos, perm, ftab, nq, dq,
-                       items, fpart);
-  else
-    hipLaunchKernelGGL((fq_kernel<M, false>), grid, dim3(256), 0, st, pos, perm, ftab, nq, dq,
-                       items, fpart);
+  hipLaunchKernelGGL((fq_kernel<M, WEIGHTED>), grid, dim3(64 * kWaves), 0, st, pos, perm, ftab,
+                     nq, dq, items, fpart);
+}
+
+hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* perm,
+                     const float* ftab, int nq, double dq, const Item* items, int n_items,
+                     bool weighted, double* fpart) {
+  if (n_items <= 0) return hipSuccess;
+  if (weighted) {
+    if (m == 64)
+      launch_fq_m<64, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 80)
+      launch_fq_m<80, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 96)
+      launch_fq_m<96, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 112)
+      launch_fq_m<112, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    if (m == 64)
+      launch_fq_m<64, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 80)
+      launch_fq_m<80, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 96)
+      launch_fq_m<96, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 112)
+      launch_fq_m<112, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -14,10 +14,10 @@ struct alignas(32) AtomRec {
   double x, y, z, w;
 };
 
-// Q bins per wave in the fused F + gradient kernel and in the F-only kernel.
+// Q bins per wave of the fused F + gradient kernel (the larger of its two shapes).
 constexpr int kGradM = 32;
-constexpr int kFqM = 64;
-constexpr int kWaves = 4;      // waves per block in the F-only and contracted-force kernels
+constexpr int kFqM = 64;  // default Q bins per wave of the F-only kernel
+constexpr int kWaves = 4;      // waves per block in the contracted-force kernel
 constexpr int kFusedWaves = 8; // waves per block in the fused F + gradient kernel
 constexpr int kJTile = 256;    // atoms staged in LDS per j tile
 
@@ -61,12 +61,13 @@ hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int
                                     const Range* tiles, int n_tiles, double conv, double* out);
 // uq[b][q] = u[b][q] * float32(q dq)
 hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
-// Unordered-pair F only: fpart[item][q] (double).  Weighted: item indices
-// are positions in perm and item.pad is the element of the item's j range;
+// Unordered-pair F only: fpart[item][q] (double), m Q bins per wave (64, 80,
+// 96 or 112).  Weighted: item indices are positions in perm, item.split is
+// the element of the item's i tile and item.pad that of its j range;
 // unweighted: atom indices.
-hipError_t launch_fq(hipStream_t st, const AtomRec* pos, const int32_t* perm, const float* ftab,
-                     int nq, double dq, const Item* items, int n_items, bool weighted,
-                     double* fpart);
+hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* perm,
+                     const float* ftab, int nq, double dq, const Item* items, int n_items,
+                     bool weighted, double* fpart);
 // out[r][q] = scale[r][q] * sum_{p in ranges[r]} part[p][q], r < n_rows (FP64)
 hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range* ranges,
                                 int n_rows, int nq, const double* scale, double* out_f64,

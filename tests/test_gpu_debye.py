@@ -163,6 +163,19 @@ def test_contracted_forces_both_kernels(ctx, qbin, nq):
     assert rel(f, f_want) < TOL
 
 
+@pytest.mark.parametrize("nq", [96, 107, 120, 600, 700])
+def test_fq_bin_counts(ctx, nq):
+    """The F-only kernel's Q tilings: 6 waves of 48 (96 bins), 56 (107, 600)
+    or 64 (120, 700) bins each, with one and with two Q groups per block
+    row; a single element and a Pt/Ni alloy, against the oracle."""
+    base = structures.fcc_nearest(150, a=3.9)
+    pos = structures.jitter(base, 0.05, seed=nq).astype(np.float32)
+    qbin = 0.05
+    for numbers in (np.full(len(pos), 79), structures.random_alloy(len(pos), 78, 28, seed=nq)):
+        per_atom = load(ctx, pos, numbers, qbin, nq)
+        assert rel(ctx.fq(_lib.SLOT_FQ, qbin, nq), oracle.fq(pos, per_atom, qbin)) < TOL
+
+
 def test_ten_elements_and_mixed_batch(ctx):
     """All ten tabulated elements in one particle, in random order: F, the
     FP64 gradient and the F(Q)-Rw forces (10 rows x 1,700 bins exceed the
