@@ -48,6 +48,7 @@ _SIGNATURES = [
     ("pyiid_synchronize", [_P]),
     ("pyiid_enable_timing", [_P, _I]),
     ("pyiid_kernel_time", [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    ("pyiid_set_fq_bins_per_wave", [_P, _I]),
     ("pyiid_set_positions", [_P, _P, _I64, _I]),
     ("pyiid_set_elements", [_P, _P, _I64, _I]),
     ("pyiid_set_scatter", [_P, _I, _P, _I32, _I32, _I]),
@@ -167,6 +168,10 @@ class Context:
 
     def enable_timing(self, on=True):
         check(self.lib.pyiid_enable_timing(self.handle, 1 if on else 0))
+
+    def set_fq_bins_per_wave(self, m):
+        """Fix the F-only kernel's Q bins per wave (64, 80, 96, 112), or 0 to autotune."""
+        check(self.lib.pyiid_set_fq_bins_per_wave(self.handle, int(m)))
 
     def kernel_time(self, kind):
         ms = ctypes.c_double()

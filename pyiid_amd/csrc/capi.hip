@@ -174,6 +174,7 @@ struct pyiid_ctx {
   // launch shapes (set by the autotuner per problem shape)
   int grad_m = 32;             // Q bins per wave of the fused kernel
   int fq_m = kFqM;             // Q bins per wave of the F-only kernel
+  int fq_m_fixed = 0;          // set by pyiid_set_fq_bins_per_wave (0: autotuned)
   int64_t fused_waves = 32768; // target wave count that sets the fused j split
   int64_t pf_waves = 8192;     // target wave count that sets the force Q groups
   bool autotune = true;
@@ -715,6 +716,10 @@ int tune_fq(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
   const std::vector<int64_t> key{2, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
                                  static_cast<int64_t>(c->struct_stamp)};
+  if (c->fq_m_fixed) {
+    c->fq_m = c->fq_m_fixed;
+    return PYIID_OK;
+  }
   auto hit = c->tuned.find(key);
   if (hit != c->tuned.end()) {
     c->fq_m = static_cast<int>(hit->second[0]);
@@ -1375,6 +1380,14 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
     c->t_ms[k] = 0.0;
     c->t_n[k] = 0;
   }
+  return PYIID_OK;
+}
+
+int pyiid_set_fq_bins_per_wave(pyiid_ctx* c, int m) {
+  CHK(check_ctx(c));
+  if (m != 0 && m != 64 && m != 80 && m != 96 && m != 112)
+    return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 64, 80, 96 or 112", m);
+  c->fq_m_fixed = m;
   return PYIID_OK;
 }
 

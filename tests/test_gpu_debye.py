@@ -176,6 +176,36 @@ def test_fq_bin_counts(ctx, nq):
         assert rel(ctx.fq(_lib.SLOT_FQ, qbin, nq), oracle.fq(pos, per_atom, qbin)) < TOL
 
 
+@pytest.mark.parametrize("m", [64, 80, 96, 112])
+def test_fq_every_bins_per_wave(ctx, m):
+    """Each launch shape of the F-only kernel (m bins per wave, fixed instead
+    of autotuned) against the oracle: a batch of Au chains at 330 bins (one Q
+    group for m >= 96, two below) and a Pt/Ni alloy at 700 bins."""
+    try:
+        ctx.set_fq_bins_per_wave(m)
+        chains = [structures.jitter(structures.fcc_nearest(n, a=4.08), 0.05, seed=n).astype(np.float32)
+                  for n in (90, 61, 2)]
+        qbin, nq = 0.0756865, 330
+        table, index, _ = element_table(np.full(sum(len(c) for c in chains), 79), qbin, nq)
+        ctx.set_batch(chains)
+        ctx.set_elements(index)
+        ctx.set_scatter(_lib.SLOT_FQ, table)
+        fq = ctx.fq(_lib.SLOT_FQ, qbin, nq, n_chain=len(chains))
+        for b, p in enumerate(chains):
+            assert rel(fq[b], oracle.fq(p, table[np.zeros(len(p), int)], qbin)) < TOL
+        pos = structures.jitter(structures.fcc_nearest(170, a=3.724), 0.05, seed=m).astype(np.float32)
+        numbers = structures.random_alloy(len(pos), 78, 28, seed=m)
+        per_atom = load(ctx, pos, numbers, 0.05, 700)
+        assert rel(ctx.fq(_lib.SLOT_FQ, 0.05, 700), oracle.fq(pos, per_atom, 0.05)) < TOL
+    finally:
+        ctx.set_fq_bins_per_wave(0)
+
+
+def test_fq_bins_per_wave_rejects_other_values(ctx):
+    with pytest.raises(ValueError, match="bins per wave"):
+        ctx.set_fq_bins_per_wave(72)
+
+
 def test_ten_elements_and_mixed_batch(ctx):
     """All ten tabulated elements in one particle, in random order: F, the
     FP64 gradient and the F(Q)-Rw forces (10 rows x 1,700 bins exceed the

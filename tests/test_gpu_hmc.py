@@ -214,7 +214,9 @@ def test_batch_with_degenerate_chain(tiny):
     en3, f3 = run([a, t, b], np.stack([gobs[0], gobs[0], gobs[1]]))
     assert np.isnan(en3[1])
     assert np.all(np.isfinite(en3[[0, 2]]))
-    np.testing.assert_allclose(en3[[0, 2]], en2, rtol=1e-9)
+    # energy = 2 Rw: the Rw bar of 1e-6 (here Rw itself is ~6e-7, the targets
+    # being the chains' own patterns)
+    np.testing.assert_allclose(en3[[0, 2]], en2, rtol=0, atol=2e-7)
     scale = np.max(np.abs(f2))
     np.testing.assert_allclose(f3[:13], f2[:13], rtol=0, atol=1e-6 * scale)
     np.testing.assert_allclose(f3[13 + tiny:], f2[13:], rtol=0, atol=1e-6 * scale)

@@ -11,7 +11,7 @@ Pins the numerical choices of the kernels:
 * the Reinsch difference form y' = y + D', D'' = D' - eps y' on the sequence
   (-1)^(k m) y with eps = 4 sin^2(d/2) (the F-only kernel's form for steps
   near a multiple of pi) is accurate there too.
-Errors are in units of the float32 ulp of 1 (2^-23) over 32 and 64 steps."""
+Errors are in units of the float32 ulp of 1 (2^-23) over 32, 64 and 112 steps."""
 import numpy as np
 import pytest
 
@@ -109,3 +109,13 @@ def test_reinsch_form_for_every_step(dth):
     # a wave that takes the Reinsch form runs it on all its lanes, whatever
     # their step: with eps rounded once from FP64 it stays within ~15 ulp
     assert worst(reinsch, dth, 64) < 20
+
+
+def test_f_only_longest_chunk():
+    # the F-only kernel runs up to 112 bins from an anchor: its three-term form
+    # (only for steps at least 0.2 from k pi) stays within ~95 ulp and its
+    # Reinsch form within ~20 for every step
+    far = [0.2, 0.25, 0.5, 1.0, 1.3, np.pi - 0.2, np.pi + 0.2, 2 * np.pi - 0.2, 2.5, 3.5, 5.0]
+    assert max(worst(two_float, d, 112) for d in far) < 120
+    anyd = [1e-3, 0.01, 0.1, 0.19, np.pi - 0.01, np.pi + 0.1, 2 * np.pi - 0.05, 0.5, 1.5, 2.9]
+    assert max(worst(reinsch, d, 112) for d in anyd) < 30
