@@ -814,14 +814,21 @@ __global__ __launch_bounds__(1024) void reduce_ranges_kernel(const double* __res
   if (out32) out32[(size_t)r * nq + q] = static_cast<float>(v);
 }
 
+// out[row][q] = gscale[q] * sum_s gpart[s][row][q] in FP64; rows are
+// (atom, component), n_split partial planes of rows x nq floats.
 template <typename T>
-__global__ void reduce_grad_kernel(const float* __restrict__ gpart, int n_split, int64_t total,
+__global__ void reduce_grad_kernel(const float* __restrict__ gpart, int n_split, int64_t rows,
                                    int nq, const double* __restrict__ gscale, T* __restrict__ out) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
+  // x: bins of a row (coalesced), y: rows (atom, component), grid-strided
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const double g = gscale[q];
+  const size_t total = static_cast<size_t>(rows) * nq;
+  for (int64_t row = blockIdx.y; row < rows; row += gridDim.y) {
+    const size_t e = static_cast<size_t>(row) * nq + q;
     double acc = 0.0;
-    for (int s = 0; s < n_split; ++s) acc += static_cast<double>(gpart[(size_t)s * total + e]);
-    out[e] = static_cast<T>(acc * gscale[e % nq]);
+    for (int s = 0; s < n_split; ++s) acc += static_cast<double>(gpart[s * total + e]);
+    out[e] = static_cast<T>(acc * g);
   }
 }
 
@@ -1034,13 +1041,15 @@ hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range*
 
 hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, int64_t n, int nq,
                               const double* gscale, void* out, bool out_f64) {
-  const int64_t total = n * 3 * nq;
+  const int64_t rows = n * 3;
+  if (rows <= 0 || nq <= 0) return hipSuccess;
+  const dim3 grid((nq + 255) / 256, static_cast<unsigned>(std::min<int64_t>(rows, 65535)));
   if (out_f64)
-    hipLaunchKernelGGL(reduce_grad_kernel<double>, dim3(grid_for(total, 256)), dim3(256), 0, st,
-                       gpart, n_split, total, nq, gscale, static_cast<double*>(out));
+    hipLaunchKernelGGL(reduce_grad_kernel<double>, grid, dim3(256), 0, st, gpart, n_split, rows,
+                       nq, gscale, static_cast<double*>(out));
   else
-    hipLaunchKernelGGL(reduce_grad_kernel<float>, dim3(grid_for(total, 256)), dim3(256), 0, st,
-                       gpart, n_split, total, nq, gscale, static_cast<float*>(out));
+    hipLaunchKernelGGL(reduce_grad_kernel<float>, grid, dim3(256), 0, st, gpart, n_split, rows,
+                       nq, gscale, static_cast<float*>(out));
   return hipGetLastError();
 }
 
