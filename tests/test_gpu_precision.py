@@ -47,3 +47,25 @@ def test_alloy_20k_f_and_grad_vs_fp64_oracle():
                                               "grad_rel": err_g, "q_bins": q_idx.tolist(),
                                               "atoms_checked": len(atoms)}}))
     assert err_f < 1e-5 and err_f_only < 1e-5 and err_g < 1e-5
+
+
+def test_gradient_rows_beyond_one_grid_column():
+    """22,500 atoms: 67,500 gradient rows (atom, component), more than one
+    launch column of the split reduction holds (65,535), so its row loop
+    wraps; the gradient of atoms on both sides of that boundary against the
+    FP64 oracle."""
+    n, qbin, nq = 22500, 0.1, 200
+    pos = structures.jitter(structures.fcc_nearest(n, a=4.08), 0.05, seed=3).astype(np.float32)
+    table, index, _ = element_table(np.full(n, 79), qbin, nq)
+    ctx = _lib.Context(0)
+    ctx.set_positions(pos)
+    ctx.set_elements(index)
+    ctx.set_scatter(_lib.SLOT_FQ, table)
+    grad = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, f64=True)
+    ctx.close()
+    q_idx = np.array([5, 60, 199])
+    atoms = np.array([0, 7, 21844, 21845, 21846, 22000, n - 1])
+    _, g_ref = native.fp64_subset(pos, index, table, qbin, q_idx, atoms,
+                                  nthreads=min(16, os.cpu_count() or 1))
+    g_dev = grad[atoms][:, :, q_idx]
+    assert float(np.max(np.abs(g_dev - g_ref)) / np.max(np.abs(g_ref))) < 1e-5
