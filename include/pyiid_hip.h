@@ -68,9 +68,9 @@ int pyiid_synchronize(pyiid_ctx *ctx);
  *       3 = pair-geometry kernel, 4 = contracted-force (unordered pair) kernel. */
 int pyiid_enable_timing(pyiid_ctx *ctx, int enable);
 int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launches);
-/* Q bins per wave of the F-only kernel: 64, 80, 96 or 112 fixes the launch
- * shape; 0 (the default) lets the autotuner time the four once per problem
- * shape and keep the fastest.  The shapes differ only in how the bins are
+/* Q bins per wave of the F-only kernel: 64, 72, 80, 88, 96 or 112 fixes the
+ * launch shape; 0 (the default) lets the autotuner time the six once per
+ * problem shape and keep the fastest.  The shapes differ only in how the bins are
  * grouped, so F agrees between them to FP32 rounding.  No counterpart in the
  * reference (its kernels have no launch-shape choice). */
 int pyiid_set_fq_bins_per_wave(pyiid_ctx *ctx, int m);

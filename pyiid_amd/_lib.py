@@ -170,7 +170,7 @@ class Context:
         check(self.lib.pyiid_enable_timing(self.handle, 1 if on else 0))
 
     def set_fq_bins_per_wave(self, m):
-        """Fix the F-only kernel's Q bins per wave (64, 80, 96, 112), or 0 to autotune."""
+        """Fix the F-only kernel's Q bins per wave (64, 72, 80, 88, 96, 112), or 0 to autotune."""
         check(self.lib.pyiid_set_fq_bins_per_wave(self.handle, int(m)))
 
     def kernel_time(self, kind):

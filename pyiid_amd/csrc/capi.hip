@@ -729,7 +729,7 @@ int tune_fq(pyiid_ctx* c, int slot, double qbin) {
   if (c->capturing || !c->autotune) return PYIID_OK;
   double best = 1e300;
   int best_m = kFqM;
-  for (int m : {64, 80, 96, 112}) {
+  for (int m : {64, 72, 80, 88, 96, 112}) {
     c->fq_m = m;
     double ms = 0.0;
     HIPCHK(launch_fq_items(c, t, qbin));
@@ -1385,8 +1385,8 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
 
 int pyiid_set_fq_bins_per_wave(pyiid_ctx* c, int m) {
   CHK(check_ctx(c));
-  if (m != 0 && m != 64 && m != 80 && m != 96 && m != 112)
-    return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 64, 80, 96 or 112", m);
+  if (m != 0 && m != 64 && m != 72 && m != 80 && m != 88 && m != 96 && m != 112)
+    return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 64, 72, 80, 88, 96 or 112", m);
   c->fq_m_fixed = m;
   return PYIID_OK;
 }

@@ -1007,8 +1007,12 @@ hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* p
   if (weighted) {
     if (m == 64)
       launch_fq_m<64, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 72)
+      launch_fq_m<72, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
     else if (m == 80)
       launch_fq_m<80, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 88)
+      launch_fq_m<88, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
     else if (m == 96)
       launch_fq_m<96, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
     else if (m == 112)
@@ -1018,8 +1022,12 @@ hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* p
   } else {
     if (m == 64)
       launch_fq_m<64, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 72)
+      launch_fq_m<72, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
     else if (m == 80)
       launch_fq_m<80, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else if (m == 88)
+      launch_fq_m<88, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
     else if (m == 96)
       launch_fq_m<96, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
     else if (m == 112)

@@ -176,11 +176,11 @@ def test_fq_bin_counts(ctx, nq):
         assert rel(ctx.fq(_lib.SLOT_FQ, qbin, nq), oracle.fq(pos, per_atom, qbin)) < TOL
 
 
-@pytest.mark.parametrize("m", [64, 80, 96, 112])
+@pytest.mark.parametrize("m", [64, 72, 80, 88, 96, 112])
 def test_fq_every_bins_per_wave(ctx, m):
     """Each launch shape of the F-only kernel (m bins per wave, fixed instead
     of autotuned) against the oracle: a batch of Au chains at 330 bins (one Q
-    group for m >= 96, two below) and a Pt/Ni alloy at 700 bins."""
+    group for m >= 88, two below) and a Pt/Ni alloy at 700 bins."""
     try:
         ctx.set_fq_bins_per_wave(m)
         chains = [structures.jitter(structures.fcc_nearest(n, a=4.08), 0.05, seed=n).astype(np.float32)
@@ -203,7 +203,7 @@ def test_fq_every_bins_per_wave(ctx, m):
 
 def test_fq_bins_per_wave_rejects_other_values(ctx):
     with pytest.raises(ValueError, match="bins per wave"):
-        ctx.set_fq_bins_per_wave(72)
+        ctx.set_fq_bins_per_wave(70)
 
 
 def test_ten_elements_and_mixed_batch(ctx):
