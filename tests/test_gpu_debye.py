@@ -93,6 +93,33 @@ def test_pair_geometry(ctx):
     np.testing.assert_array_equal(r2, r[100:300])
 
 
+def test_pair_geometry_ranges_and_alignment(ctx):
+    """Pair ranges of several lengths whose ends cross rows of the triangle,
+    in host mode and into device buffers at 16-byte aligned and unaligned
+    addresses: all equal to the full array, which matches the oracle."""
+    import torch
+    pos = structures.jitter(structures.fcc_nearest(97, a=4.08), 0.05, seed=9).astype(np.float32)
+    load(ctx, pos, np.full(len(pos), 79), 0.1, 250)
+    k = len(pos) * (len(pos) - 1) // 2
+    d, r = ctx.pair_geometry(0, k)
+    dw, rw = oracle.pair_geometry(pos)
+    np.testing.assert_allclose(d, dw, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(r, rw, rtol=1e-6)
+    for off, cnt in [(0, 1), (1, 2), (5, 3), (44, 4), (45, 5), (1000, 1001), (k - 7, 7), (k - 1, 1)]:
+        d2, r2 = ctx.pair_geometry(off, cnt)
+        np.testing.assert_array_equal(d2, d[off:off + cnt])
+        np.testing.assert_array_equal(r2, r[off:off + cnt])
+    for shift in (0, 1):  # floats: 0 keeps 16-byte alignment, 1 breaks it
+        dd = torch.zeros(3 * 1001 + 4, dtype=torch.float32, device="cuda")
+        rr = torch.zeros(1001 + 4, dtype=torch.float32, device="cuda")
+        _lib.check(ctx.lib.pyiid_pair_geometry(ctx.handle, 1000, 1001, dd[shift:].data_ptr(),
+                                               rr[shift:].data_ptr(), _lib.PYIID_DEVICE))
+        ctx.synchronize()
+        np.testing.assert_array_equal(dd[shift:shift + 3 * 1001].cpu().numpy().reshape(-1, 3),
+                                      d[1000:2001])
+        np.testing.assert_array_equal(rr[shift:shift + 1001].cpu().numpy(), r[1000:2001])
+
+
 @pytest.mark.parametrize("n", [0, 1, 2])
 def test_tiny_structures(ctx, n):
     pos = np.array([[0, 0, 0], [2.5, 0.3, 0.1]], np.float32)[:n]
