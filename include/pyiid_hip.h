@@ -74,6 +74,13 @@ int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launc
  * grouped, so F agrees between them to FP32 rounding.  No counterpart in the
  * reference (its kernels have no launch-shape choice). */
 int pyiid_set_fq_bins_per_wave(pyiid_ctx *ctx, int m);
+/* Autotuning of the Debye kernels' launch shapes (fused kernel bins per wave
+ * and j split, F-only bins per wave, contracted-force Q groups), on by
+ * default: the first call for a problem shape times the candidates and keeps
+ * the fastest.  Off (enable = 0): every call uses fixed default shapes, so
+ * results are the same bit for bit from run to run and context to context;
+ * with it on they agree to FP32 rounding.  No counterpart in the reference. */
+int pyiid_set_autotune(pyiid_ctx *ctx, int enable);
 
 /* ---- structure and tables ---------------------------------------------- */
 /* Atom positions, float32 N x 3 (the reference casts positions to float32:

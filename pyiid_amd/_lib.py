@@ -49,6 +49,7 @@ _SIGNATURES = [
     ("pyiid_enable_timing", [_P, _I]),
     ("pyiid_kernel_time", [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
     ("pyiid_set_fq_bins_per_wave", [_P, _I]),
+    ("pyiid_set_autotune", [_P, _I]),
     ("pyiid_set_positions", [_P, _P, _I64, _I]),
     ("pyiid_set_elements", [_P, _P, _I64, _I]),
     ("pyiid_set_scatter", [_P, _I, _P, _I32, _I32, _I]),
@@ -168,6 +169,11 @@ class Context:
 
     def enable_timing(self, on=True):
         check(self.lib.pyiid_enable_timing(self.handle, 1 if on else 0))
+
+    def set_autotune(self, on=True):
+        """Autotune launch shapes per problem shape (default), or use fixed
+        shapes (bit-reproducible results)."""
+        check(self.lib.pyiid_set_autotune(self.handle, 1 if on else 0))
 
     def set_fq_bins_per_wave(self, m):
         """Fix the F-only kernel's Q bins per wave (64, 72, 80, 88, 96, 112), or 0 to autotune."""

@@ -24,6 +24,12 @@ using namespace pyiid;
 
 namespace {
 
+// Launch shapes used without the autotuner (pyiid_set_autotune(ctx, 0)):
+// target wave counts for the fused kernel's j split and the contracted-force
+// kernel's Q groups.
+constexpr int64_t kDefaultFusedWaves = 32768;
+constexpr int64_t kDefaultPfWaves = 8192;
+
 thread_local char g_err[1024] = "";
 
 int fail(int code, const char* fmt, ...) {
@@ -172,11 +178,11 @@ struct pyiid_ctx {
   int pf_groups = 1, pf_span = 32;
 
   // launch shapes (set by the autotuner per problem shape)
-  int grad_m = 32;             // Q bins per wave of the fused kernel
-  int fq_m = kFqM;             // Q bins per wave of the F-only kernel
-  int fq_m_fixed = 0;          // set by pyiid_set_fq_bins_per_wave (0: autotuned)
-  int64_t fused_waves = 32768; // target wave count that sets the fused j split
-  int64_t pf_waves = 8192;     // target wave count that sets the force Q groups
+  int grad_m = kGradM;                     // Q bins per wave of the fused kernel
+  int fq_m = kFqM;                         // Q bins per wave of the F-only kernel
+  int fq_m_fixed = 0;                      // set by pyiid_set_fq_bins_per_wave (0: autotuned)
+  int64_t fused_waves = kDefaultFusedWaves;  // target wave count that sets the fused j split
+  int64_t pf_waves = kDefaultPfWaves;        // target wave count that sets the force Q groups
   bool autotune = true;
   std::map<std::vector<int64_t>, std::vector<int64_t>> tuned;
 
@@ -683,6 +689,10 @@ int time_launch(pyiid_ctx* c, F fn, double* ms) {
 // shape once to warm it and once timed, keeps the fastest, and remembers it
 // for the shape.  Never runs while a graph is captured (defaults apply).
 int tune_contract(pyiid_ctx* c, int slot, double qbin) {
+  if (!c->autotune) {
+    c->pf_waves = kDefaultPfWaves;
+    return PYIID_OK;
+  }
   Table& t = c->tab[slot];
   const std::vector<int64_t> key{1, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
                                  static_cast<int64_t>(c->struct_stamp)};
@@ -691,7 +701,7 @@ int tune_contract(pyiid_ctx* c, int slot, double qbin) {
     c->pf_waves = hit->second[0];
     return PYIID_OK;
   }
-  if (c->capturing || !c->autotune) return PYIID_OK;
+  if (c->capturing) return PYIID_OK;
   double best = 1e300;
   int64_t best_w = c->pf_waves;
   for (int64_t w : {2048LL, 8192LL, 32768LL}) {
@@ -716,8 +726,8 @@ int tune_fq(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
   const std::vector<int64_t> key{2, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
                                  static_cast<int64_t>(c->struct_stamp)};
-  if (c->fq_m_fixed) {
-    c->fq_m = c->fq_m_fixed;
+  if (c->fq_m_fixed || !c->autotune) {
+    c->fq_m = c->fq_m_fixed ? c->fq_m_fixed : kFqM;
     return PYIID_OK;
   }
   auto hit = c->tuned.find(key);
@@ -726,7 +736,7 @@ int tune_fq(pyiid_ctx* c, int slot, double qbin) {
     return PYIID_OK;
   }
   c->fq_m = kFqM;
-  if (c->capturing || !c->autotune) return PYIID_OK;
+  if (c->capturing) return PYIID_OK;
   double best = 1e300;
   int best_m = kFqM;
   for (int m : {64, 72, 80, 88, 96, 112}) {
@@ -748,6 +758,11 @@ int tune_fq(pyiid_ctx* c, int slot, double qbin) {
 }
 
 int tune_fused(pyiid_ctx* c, int slot, double qbin) {
+  if (!c->autotune) {
+    c->grad_m = kGradM;
+    c->fused_waves = kDefaultFusedWaves;
+    return PYIID_OK;
+  }
   Table& t = c->tab[slot];
   const std::vector<int64_t> key{0, c->n, c->n_chain, t.nq, weighted_table(t) ? 1 : 0,
                                  static_cast<int64_t>(c->struct_stamp)};
@@ -757,7 +772,7 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
     c->fused_waves = hit->second[1];
     return PYIID_OK;
   }
-  if (c->capturing || !c->autotune) return PYIID_OK;
+  if (c->capturing) return PYIID_OK;
   double best = 1e300;
   int best_m = c->grad_m;
   int64_t best_w = c->fused_waves;
@@ -1380,6 +1395,12 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
     c->t_ms[k] = 0.0;
     c->t_n[k] = 0;
   }
+  return PYIID_OK;
+}
+
+int pyiid_set_autotune(pyiid_ctx* c, int enable) {
+  CHK(check_ctx(c));
+  c->autotune = enable != 0;
   return PYIID_OK;
 }
 

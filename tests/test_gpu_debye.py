@@ -233,6 +233,37 @@ def test_fq_bins_per_wave_rejects_other_values(ctx):
         ctx.set_fq_bins_per_wave(70)
 
 
+def test_fixed_launch_shapes_repeat_bit_for_bit():
+    """With the autotuner off, two fresh contexts give the same F, gradient,
+    energy and forces bit for bit; with it on, the same to FP32 rounding."""
+    base = structures.fcc_nearest(300, a=3.724)
+    pos = structures.jitter(base, 0.05, seed=21).astype(np.float32)
+    numbers = structures.random_alloy(len(pos), 78, 28, seed=21)
+    qbin, nq, q_first = 0.05, 500, 10
+    table, index, _ = element_table(numbers, qbin, nq)
+    target = oracle.fq((base * np.float32(1.002)).astype(np.float32), table[index], qbin)[q_first:]
+
+    def run(autotune):
+        c = _lib.Context(0)
+        try:
+            c.set_autotune(autotune)
+            load(c, pos, numbers, qbin, nq)
+            f = c.fq(_lib.SLOT_FQ, qbin, nq)
+            g = c.grad_fq(_lib.SLOT_FQ, qbin, len(pos), nq, f64=True)
+            e, fo = c.fq_energy_forces(_lib.SLOT_FQ, qbin, q_first, target, _lib.POT_RW, 10.0,
+                                       len(pos))
+            return f, g, e, fo
+        finally:
+            c.close()
+
+    a, b, t = run(False), run(False), run(True)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert rel(a[0], oracle.fq(pos, table[index], qbin)) < TOL
+    assert rel(t[0], a[0]) < 1e-6 and rel(t[1], a[1]) < 1e-6
+    assert abs(t[2] - a[2]) < 1e-6 and rel(t[3], a[3]) < 1e-5
+
+
 def test_ten_elements_and_mixed_batch(ctx):
     """All ten tabulated elements in one particle, in random order: F, the
     FP64 gradient and the F(Q)-Rw forces (10 rows x 1,700 bins exceed the
