@@ -814,6 +814,28 @@ __global__ __launch_bounds__(1024) void reduce_ranges_kernel(const double* __res
   if (out32) out32[(size_t)r * nq + q] = static_cast<float>(v);
 }
 
+// The same two bins per thread (8-byte loads), for even nq.
+template <typename T>
+__global__ void reduce_grad2_kernel(const float* __restrict__ gpart, int n_split, int64_t rows,
+                                    int nq, const double* __restrict__ gscale,
+                                    T* __restrict__ out) {
+  const int q = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= nq) return;
+  const double g0 = gscale[q], g1 = gscale[q + 1];
+  const size_t total = static_cast<size_t>(rows) * nq;
+  for (int64_t row = blockIdx.y; row < rows; row += gridDim.y) {
+    const size_t e = static_cast<size_t>(row) * nq + q;
+    double a0 = 0.0, a1 = 0.0;
+    for (int s = 0; s < n_split; ++s) {
+      const float2 v = *reinterpret_cast<const float2*>(gpart + s * total + e);
+      a0 += static_cast<double>(v.x);
+      a1 += static_cast<double>(v.y);
+    }
+    out[e] = static_cast<T>(a0 * g0);
+    out[e + 1] = static_cast<T>(a1 * g1);
+  }
+}
+
 // out[row][q] = gscale[q] * sum_s gpart[s][row][q] in FP64; rows are
 // (atom, component), n_split partial planes of rows x nq floats.
 template <typename T>
@@ -1051,7 +1073,18 @@ hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, i
                               const double* gscale, void* out, bool out_f64) {
   const int64_t rows = n * 3;
   if (rows <= 0 || nq <= 0) return hipSuccess;
-  const dim3 grid((nq + 255) / 256, static_cast<unsigned>(std::min<int64_t>(rows, 65535)));
+  const unsigned grid_y = static_cast<unsigned>(std::min<int64_t>(rows, 65535));
+  if (nq % 2 == 0 && reinterpret_cast<uintptr_t>(gpart) % 8 == 0) {
+    const dim3 grid2((nq / 2 + 255) / 256, grid_y);
+    if (out_f64)
+      hipLaunchKernelGGL(reduce_grad2_kernel<double>, grid2, dim3(256), 0, st, gpart, n_split,
+                         rows, nq, gscale, static_cast<double*>(out));
+    else
+      hipLaunchKernelGGL(reduce_grad2_kernel<float>, grid2, dim3(256), 0, st, gpart, n_split,
+                         rows, nq, gscale, static_cast<float*>(out));
+    return hipGetLastError();
+  }
+  const dim3 grid((nq + 255) / 256, grid_y);
   if (out_f64)
     hipLaunchKernelGGL(reduce_grad_kernel<double>, grid, dim3(256), 0, st, gpart, n_split, rows,
                        nq, gscale, static_cast<double*>(out));
