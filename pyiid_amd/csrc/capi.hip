@@ -328,8 +328,9 @@ int upload_items(pyiid_ctx*, ItemSet& s, const std::vector<Item>& v, const std::
 
 // Unordered-pair items: per chain, per i tile of at most 64 atoms, j ranges
 // of the chain below the tile's end.  Ranges: one per chain.  Item indices
-// are positions in perm, the chain's atoms in element order (stable) for a
-// weighted table and in their own order otherwise; for a weighted table no i
+// are positions in perm, the chain's atoms in Z-order of their positions when
+// the items are built (after element order for a weighted table); for a
+// weighted table no i
 // tile or j range crosses an element boundary, item.split is the i tile's
 // element and item.pad the j range's, so that the kernel sums plain
 // sin(Q r)/r and weights each bin's total by f_i(Q) f_j(Q) once.
@@ -343,6 +344,45 @@ int ensure_fq_items(pyiid_ctx* c, bool weighted) {
   std::vector<int32_t> perm(static_cast<size_t>(c->n));
   for (int64_t a = 0; a < c->n; ++a) perm[a] = static_cast<int32_t>(a);
   auto el = [&](int32_t a) { return c->elem_h.empty() ? 0 : c->elem_h[a]; };
+  // Morton (Z-order) code of every atom over its chain's bounding box, from the
+  // packed positions: a 64-atom i tile in this order is compact in space, so
+  // the distances from one j atom to its atoms span a narrow range and fewer
+  // waves meet a step near a multiple of pi (the slower Reinsch form).
+  std::vector<uint32_t> zkey(static_cast<size_t>(c->n), 0u);
+  if (c->n > 0) {
+    std::vector<AtomRec> h(static_cast<size_t>(c->n));
+    HIPCHK(hipMemcpyAsync(h.data(), c->pos.p, sizeof(AtomRec) * h.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    auto spread = [](uint32_t x) {  // 10 bits -> every third of 30
+      x &= 0x3ffu;
+      x = (x | (x << 16)) & 0x030000ffu;
+      x = (x | (x << 8)) & 0x0300f00fu;
+      x = (x | (x << 4)) & 0x030c30c3u;
+      x = (x | (x << 2)) & 0x09249249u;
+      return x;
+    };
+    for (int b = 0; b < c->n_chain; ++b) {
+      const int64_t o = c->off[b], e = c->off[b + 1];
+      if (e <= o) continue;
+      double lo[3] = {h[o].x, h[o].y, h[o].z}, hi[3] = {h[o].x, h[o].y, h[o].z};
+      for (int64_t a = o; a < e; ++a) {
+        const double p3[3] = {h[a].x, h[a].y, h[a].z};
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], p3[k]);
+          hi[k] = std::max(hi[k], p3[k]);
+        }
+      }
+      const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-30});
+      for (int64_t a = o; a < e; ++a) {
+        const double p3[3] = {h[a].x, h[a].y, h[a].z};
+        uint32_t q[3];
+        for (int k = 0; k < 3; ++k)
+          q[k] = static_cast<uint32_t>(std::min(1023.0, std::max(0.0, (p3[k] - lo[k]) / span * 1023.0)));
+        zkey[a] = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
+      }
+    }
+  }
   int tile = 0;
   for (int b = 0; b < c->n_chain; ++b) {
     const int64_t o = c->off[b], nb = c->off[b + 1] - o;
@@ -350,9 +390,12 @@ int ensure_fq_items(pyiid_ctx* c, bool weighted) {
     const int64_t jc = 256 * std::max<int64_t>(1, nb / (256 * 16));
     // seg_end[p - o]: end of the element run holding position p
     std::vector<int64_t> seg_end(static_cast<size_t>(nb), o + nb);
+    // element order (weighted tables), then Z-order; ties keep atom order
+    std::stable_sort(perm.begin() + o, perm.begin() + o + nb, [&](int32_t x, int32_t y) {
+      const int ex = weighted ? el(x) : 0, ey = weighted ? el(y) : 0;
+      return ex != ey ? ex < ey : zkey[x] < zkey[y];
+    });
     if (weighted) {
-      std::stable_sort(perm.begin() + o, perm.begin() + o + nb,
-                       [&](int32_t x, int32_t y) { return el(x) < el(y); });
       for (int64_t p = o + nb - 2; p >= o; --p)
         seg_end[p - o] = el(perm[p]) == el(perm[p + 1]) ? seg_end[p + 1 - o] : p + 1;
     }

@@ -400,12 +400,11 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
 // F-only kernel (unordered pairs j < i), Chebyshev recurrence on omega.
 //   grid: x = work item (i tile, j range of the same chain), y = Q group
 //   fpart[item][q] double
-// Weighted: item indices are positions in perm, the atoms in element order;
-// the item's i tile is of one element, item.split, and its j range of one
-// element, item.pad, so that F = sum_{e,e'} f_e f_e' S_ee' holds per item:
-// the loop sums plain sin(Q r)/r and each bin's total is weighted by
-// f_i(Q) f_j(Q) at the end.
-// Unweighted: item indices are atom indices (perm is not read).
+// Item indices are positions in perm (the atoms in Z-order, after element
+// order for a weighted table).  Weighted: the item's i tile is of one element,
+// item.split, and its j range of one element, item.pad, so that
+// F = sum_{e,e'} f_e f_e' S_ee' holds per item: the loop sums plain
+// sin(Q r)/r and each bin's total is weighted by f_i(Q) f_j(Q) at the end.
 // ---------------------------------------------------------------------------
 // F-only inner loops over NP j atoms of a round at once (their LDS offsets
 // o[p]), adding sin(Q_m r)/r of each pair to acc[m] for the wave's M bins.
@@ -506,8 +505,7 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
   const int chunk = (wave + static_cast<int>((blockIdx.x * 0x9E3779B1u) >> 30)) & (kWaves - 1);
   const int q0 = (blockIdx.y * kWaves + chunk) * M;
   const bool lane_ok = i < it.i_end;
-  const int ii = lane_ok ? i : it.i_base;
-  const AtomRec pi = pos[WEIGHTED ? perm[ii] : ii];
+  const AtomRec pi = pos[perm[lane_ok ? i : it.i_base]];
 
   float acc[M];
 #pragma unroll
@@ -518,7 +516,7 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
   for (int jt = it.jbeg; jt < it.jend; jt += kJTile) {
     __syncthreads();
     if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < it.jend)
-      tile[threadIdx.x] = pos[WEIGHTED ? perm[jt + threadIdx.x] : jt + threadIdx.x];
+      tile[threadIdx.x] = pos[perm[jt + threadIdx.x]];
     __syncthreads();
     const int cnt = min(kJTile, it.jend - jt);
     for (int js = 0; js < cnt; js += kFqGeoJ) {

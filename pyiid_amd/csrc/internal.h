@@ -62,9 +62,9 @@ hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int
 // uq[b][q] = u[b][q] * float32(q dq)
 hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
 // Unordered-pair F only: fpart[item][q] (double), m Q bins per wave (64, 72,
-// 80, 88, 96 or 112).  Weighted: item indices are positions in perm, item.split is
-// the element of the item's i tile and item.pad that of its j range;
-// unweighted: atom indices.
+// 80, 88, 96 or 112).  Item indices are positions in perm; weighted:
+// item.split is the element of the item's i tile and item.pad that of its j
+// range.
 hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* perm,
                      const float* ftab, int nq, double dq, const Item* items, int n_items,
                      bool weighted, double* fpart);
