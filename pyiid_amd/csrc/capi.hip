@@ -187,7 +187,7 @@ struct pyiid_ctx {
   std::map<std::vector<int64_t>, std::vector<int64_t>> tuned;
 
   // workspaces
-  Buf gpart, fpart, forcepart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, uq, gobs, pfpart;
+  Buf gpart, fpart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, uq, gobs, pfpart;
 
   Hmc hmc;
   Sur sur;
@@ -326,90 +326,99 @@ int upload_items(pyiid_ctx*, ItemSet& s, const std::vector<Item>& v, const std::
   return PYIID_OK;
 }
 
+// Order of every chain's atoms for the unordered-pair work lists, as
+// positions -> atom indices (perm): element order for a weighted table, then
+// the Z-order (Morton code over the chain's bounding box) of the positions
+// when the lists are built, ties in atom order.  A 64-atom tile in Z-order is
+// compact in space, so the distances from one j atom to its atoms span a
+// narrow range and fewer F-only waves meet a step near a multiple of pi (the
+// slower Reinsch form).  seg_end[p]: end of the element run holding position
+// p (the chain's end when unweighted).
+int order_atoms(pyiid_ctx* c, bool weighted, std::vector<int32_t>& perm,
+                std::vector<int64_t>& seg_end) {
+  perm.resize(static_cast<size_t>(c->n));
+  seg_end.resize(static_cast<size_t>(c->n));
+  for (int64_t a = 0; a < c->n; ++a) perm[a] = static_cast<int32_t>(a);
+  if (c->n == 0) return PYIID_OK;
+  auto el = [&](int32_t a) { return c->elem_h.empty() ? 0 : c->elem_h[a]; };
+  std::vector<AtomRec> h(static_cast<size_t>(c->n));
+  HIPCHK(hipMemcpyAsync(h.data(), c->pos.p, sizeof(AtomRec) * h.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto spread = [](uint32_t x) {  // 10 bits -> every third of 30
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+  };
+  std::vector<uint32_t> zkey(static_cast<size_t>(c->n), 0u);
+  for (int b = 0; b < c->n_chain; ++b) {
+    const int64_t o = c->off[b], e = c->off[b + 1];
+    if (e <= o) continue;
+    double lo[3] = {h[o].x, h[o].y, h[o].z}, hi[3] = {h[o].x, h[o].y, h[o].z};
+    for (int64_t a = o; a < e; ++a) {
+      const double p3[3] = {h[a].x, h[a].y, h[a].z};
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], p3[k]);
+        hi[k] = std::max(hi[k], p3[k]);
+      }
+    }
+    const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-30});
+    for (int64_t a = o; a < e; ++a) {
+      const double p3[3] = {h[a].x, h[a].y, h[a].z};
+      uint32_t q[3];
+      for (int k = 0; k < 3; ++k)
+        q[k] = static_cast<uint32_t>(std::min(1023.0, std::max(0.0, (p3[k] - lo[k]) / span * 1023.0)));
+      zkey[a] = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
+    }
+    std::stable_sort(perm.begin() + o, perm.begin() + e, [&](int32_t x, int32_t y) {
+      const int ex = weighted ? el(x) : 0, ey = weighted ? el(y) : 0;
+      return ex != ey ? ex < ey : zkey[x] < zkey[y];
+    });
+    seg_end[e - 1] = e;
+    for (int64_t p = e - 2; p >= o; --p)
+      seg_end[p] = (!weighted || el(perm[p]) == el(perm[p + 1])) ? seg_end[p + 1] : p + 1;
+  }
+  return PYIID_OK;
+}
+
+int upload_perm(pyiid_ctx*, ItemSet& s, const std::vector<int32_t>& perm) {
+  CHK(s.perm.reserve(sizeof(int32_t) * std::max<size_t>(1, perm.size())));
+  if (!perm.empty())
+    HIPCHK(hipMemcpy(s.perm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+  return PYIID_OK;
+}
+
 // Unordered-pair items: per chain, per i tile of at most 64 atoms, j ranges
 // of the chain below the tile's end.  Ranges: one per chain.  Item indices
-// are positions in perm, the chain's atoms in Z-order of their positions when
-// the items are built (after element order for a weighted table); for a
-// weighted table no i
-// tile or j range crosses an element boundary, item.split is the i tile's
-// element and item.pad the j range's, so that the kernel sums plain
-// sin(Q r)/r and weights each bin's total by f_i(Q) f_j(Q) once.
+// are positions in perm (order_atoms); for a weighted table no i tile or j
+// range crosses an element boundary, item.split is the i tile's element and
+// item.pad the j range's, so that the kernel sums plain sin(Q r)/r and
+// weights each bin's total by f_i(Q) f_j(Q) once.
 int ensure_fq_items(pyiid_ctx* c, bool weighted) {
   ItemSet& s = c->fq_items;
   const int key = weighted ? 1 : 0;
   if (s.stamp == c->struct_stamp && s.key == key) return PYIID_OK;
   if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  std::vector<int32_t> perm;
+  std::vector<int64_t> seg_end;
+  CHK(order_atoms(c, weighted, perm, seg_end));
+  auto el = [&](int32_t a) { return c->elem_h.empty() ? 0 : c->elem_h[a]; };
   std::vector<Item> v;
   std::vector<Range> r;
-  std::vector<int32_t> perm(static_cast<size_t>(c->n));
-  for (int64_t a = 0; a < c->n; ++a) perm[a] = static_cast<int32_t>(a);
-  auto el = [&](int32_t a) { return c->elem_h.empty() ? 0 : c->elem_h[a]; };
-  // Morton (Z-order) code of every atom over its chain's bounding box, from the
-  // packed positions: a 64-atom i tile in this order is compact in space, so
-  // the distances from one j atom to its atoms span a narrow range and fewer
-  // waves meet a step near a multiple of pi (the slower Reinsch form).
-  std::vector<uint32_t> zkey(static_cast<size_t>(c->n), 0u);
-  if (c->n > 0) {
-    std::vector<AtomRec> h(static_cast<size_t>(c->n));
-    HIPCHK(hipMemcpyAsync(h.data(), c->pos.p, sizeof(AtomRec) * h.size(), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    auto spread = [](uint32_t x) {  // 10 bits -> every third of 30
-      x &= 0x3ffu;
-      x = (x | (x << 16)) & 0x030000ffu;
-      x = (x | (x << 8)) & 0x0300f00fu;
-      x = (x | (x << 4)) & 0x030c30c3u;
-      x = (x | (x << 2)) & 0x09249249u;
-      return x;
-    };
-    for (int b = 0; b < c->n_chain; ++b) {
-      const int64_t o = c->off[b], e = c->off[b + 1];
-      if (e <= o) continue;
-      double lo[3] = {h[o].x, h[o].y, h[o].z}, hi[3] = {h[o].x, h[o].y, h[o].z};
-      for (int64_t a = o; a < e; ++a) {
-        const double p3[3] = {h[a].x, h[a].y, h[a].z};
-        for (int k = 0; k < 3; ++k) {
-          lo[k] = std::min(lo[k], p3[k]);
-          hi[k] = std::max(hi[k], p3[k]);
-        }
-      }
-      const double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-30});
-      for (int64_t a = o; a < e; ++a) {
-        const double p3[3] = {h[a].x, h[a].y, h[a].z};
-        uint32_t q[3];
-        for (int k = 0; k < 3; ++k)
-          q[k] = static_cast<uint32_t>(std::min(1023.0, std::max(0.0, (p3[k] - lo[k]) / span * 1023.0)));
-        zkey[a] = spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2);
-      }
-    }
-  }
   int tile = 0;
   for (int b = 0; b < c->n_chain; ++b) {
     const int64_t o = c->off[b], nb = c->off[b + 1] - o;
     const int begin = static_cast<int>(v.size());
     const int64_t jc = 256 * std::max<int64_t>(1, nb / (256 * 16));
-    // seg_end[p - o]: end of the element run holding position p
-    std::vector<int64_t> seg_end(static_cast<size_t>(nb), o + nb);
-    // element order (weighted tables), then Z-order; ties keep atom order
-    std::stable_sort(perm.begin() + o, perm.begin() + o + nb, [&](int32_t x, int32_t y) {
-      const int ex = weighted ? el(x) : 0, ey = weighted ? el(y) : 0;
-      return ex != ey ? ex < ey : zkey[x] < zkey[y];
-    });
-    if (weighted) {
-      for (int64_t p = o + nb - 2; p >= o; --p)
-        seg_end[p - o] = el(perm[p]) == el(perm[p + 1]) ? seg_end[p + 1 - o] : p + 1;
-    }
     for (int64_t ib = o, ie = o; ib < o + nb; ib = ie, ++tile) {
-      ie = std::min(o + nb, ib + 64);
-      if (weighted) ie = std::min(ie, seg_end[ib - o]);
+      ie = std::min(seg_end[ib], ib + 64);
       const int ei = weighted ? el(perm[ib]) : 0;
       for (int64_t jb = o; jb < ie;) {
-        int64_t je = std::min(ie, jb + jc);
-        int ej = 0;
-        if (weighted) {
-          je = std::min(je, seg_end[jb - o]);
-          ej = el(perm[jb]);
-        }
+        const int64_t je = std::min({ie, jb + jc, seg_end[jb]});
+        const int ej = weighted ? el(perm[jb]) : 0;
         v.push_back(Item{static_cast<int>(ib), static_cast<int>(ie), static_cast<int>(jb),
                          static_cast<int>(je), ei, b, tile, ej});
         jb = je;
@@ -419,9 +428,7 @@ int ensure_fq_items(pyiid_ctx* c, bool weighted) {
                       static_cast<int>(o + nb)});
   }
   CHK(upload_items(c, s, v, r));
-  CHK(s.perm.reserve(sizeof(int32_t) * std::max<size_t>(1, perm.size())));
-  if (!perm.empty())
-    HIPCHK(hipMemcpy(s.perm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice));
+  CHK(upload_perm(c, s, perm));
   s.stamp = c->struct_stamp;
   s.key = key;
   return PYIID_OK;
@@ -597,32 +604,25 @@ int compute_fq(pyiid_ctx* c, int slot, double qbin) {
   return PYIID_OK;
 }
 
-// Fused kernel over all chains: gradient partials (u == null) or contracted
-// force partials, plus F partials.
+// Fused kernel over all chains: gradient partials and F partials.
 int tune_fused(pyiid_ctx* c, int slot, double qbin);
 
-int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, bool tune = true) {
+int run_fused(pyiid_ctx* c, int slot, double qbin, bool tune = true) {
   Table& t = c->tab[slot];
   const int nq = t.nq;
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
-  if (tune && !u) CHK(tune_fused(c, slot, qbin));
+  if (tune) CHK(tune_fused(c, slot, qbin));
   CHK(ensure_fused_items(c, nq, weighted_table(t)));
   const ItemSet& s = c->fused_items;
-  const int n_qchunk =
-      ((nq + kFusedWaves * c->grad_m - 1) / (kFusedWaves * c->grad_m)) * kFusedWaves;
   CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, s.n_items) * nq));
-  if (u) {
-    CHK(c->forcepart.reserve(sizeof(float) * (size_t)std::max(1, s.n_items) * n_qchunk * 64 * 3));
-  } else {
-    CHK(c->gpart.reserve(sizeof(float) * (size_t)s.split * std::max<int64_t>(1, c->n) * 3 * nq));
-  }
+  CHK(c->gpart.reserve(sizeof(float) * (size_t)s.split * std::max<int64_t>(1, c->n) * 3 * nq));
   const double dq = static_cast<double>(static_cast<float>(qbin));
   Timed tm(c, 0);
   HIPCHK(launch_fused_grad(c->stream, c->grad_m, c->pos.as<AtomRec>(), s.perm.as<int32_t>(),
                            static_cast<int>(c->n), t.tab.as<float>(), nq, dq, s.items.as<Item>(),
                            s.n_items, weighted_table(t), c->gpart.as<float>(),
-                           c->fpart.as<double>(), u, c->forcepart.as<float>()));
+                           c->fpart.as<double>()));
   CHK(tm.done());
   return PYIID_OK;
 }
@@ -630,7 +630,9 @@ int run_fused(pyiid_ctx* c, int slot, double qbin, const float* u, bool tune = t
 // Tile pairs (I, J <= I) of every chain for the contracted-force kernel, the
 // Q grouping (enough waves to fill the device, LDS within its limit) and the
 // (work, side) entry list of every tile for the deterministic reduction.
-int ensure_pair_items(pyiid_ctx* c, int nq, bool weighted, int n_elem) {
+// Tiles are runs of at most 64 positions in order_atoms order; for a weighted
+// table they do not cross element boundaries and item.split = e_I | e_J << 16.
+int ensure_pair_items(pyiid_ctx* c, int nq, bool weighted) {
   ItemSet& s = c->pair_items;
   int64_t n_items = 0;
   for (int b = 0; b < c->n_chain; ++b) {
@@ -640,36 +642,42 @@ int ensure_pair_items(pyiid_ctx* c, int nq, bool weighted, int n_elem) {
   const int n_chunks = std::max(1, (nq + 31) / 32);
   int64_t groups = (c->pf_waves + std::max<int64_t>(1, n_items) - 1) / std::max<int64_t>(1, n_items);
   groups = std::max<int64_t>(1, std::min<int64_t>(groups, n_chunks));
-  const size_t table = weighted ? sizeof(float) * static_cast<size_t>(n_elem) * nq : 0;
   int span = 0;
   for (;; ++groups) {
     const int per = static_cast<int>((n_chunks + groups - 1) / groups);
     span = per * 32;
-    if (sizeof(float) * 2 * kWaves * static_cast<size_t>(span) + table <= kPairForceLds) break;
+    if (sizeof(float) * 2 * kWaves * static_cast<size_t>(span) <= kPairForceLds) break;
     if (groups >= n_chunks) return fail(PYIID_E_INVALID, "contracted-force tables exceed LDS");
   }
   groups = (n_chunks + span / 32 - 1) / (span / 32);
-  if (n_items * groups > INT32_MAX / 2) return fail(PYIID_E_INVALID, "too many atom pairs");
-  const int key = static_cast<int>(groups * 100000 + span);
+  const int key = static_cast<int>(groups * 100000 + span) * 2 + (weighted ? 1 : 0);
   if (s.stamp == c->struct_stamp && s.key == key) return PYIID_OK;
   if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  std::vector<int32_t> perm;
+  std::vector<int64_t> seg_end;
+  CHK(order_atoms(c, weighted, perm, seg_end));
+  auto el = [&](int64_t p) { return weighted && !c->elem_h.empty() ? c->elem_h[perm[p]] : 0; };
   std::vector<Item> v;
   std::vector<std::vector<int>> per_tile;
   std::vector<Range> tiles;
   int tile0 = 0;
   for (int b = 0; b < c->n_chain; ++b) {
-    const int o = static_cast<int>(c->off[b]), e = static_cast<int>(c->off[b + 1]);
-    const int nt = (e - o + 63) / 64;
-    for (int t = 0; t < nt; ++t) {
-      tiles.push_back(Range{0, 0, o + 64 * t, std::min(e, o + 64 * (t + 1))});
+    const int64_t o = c->off[b], e = c->off[b + 1];
+    std::vector<std::pair<int, int>> run;  // the chain's tiles [lo, hi)
+    for (int64_t p = o; p < e;) {
+      const int64_t q = std::min(p + 64, seg_end[p]);
+      run.emplace_back(static_cast<int>(p), static_cast<int>(q));
+      tiles.push_back(Range{0, 0, static_cast<int>(p), static_cast<int>(q)});
       per_tile.emplace_back();
+      p = q;
     }
+    const int nt = static_cast<int>(run.size());
     for (int ti = 0; ti < nt; ++ti)
       for (int tj = 0; tj <= ti; ++tj) {
-        const int ib = o + 64 * ti, jb = o + 64 * tj;
         const int k = static_cast<int>(v.size());
-        v.push_back(Item{ib, std::min(e, ib + 64), jb, std::min(e, jb + 64), 0, b, tile0 + ti,
-                         tile0 + tj});
+        const int es = el(run[ti].first) | (el(run[tj].first) << 16);
+        v.push_back(Item{run[ti].first, run[ti].second, run[tj].first, run[tj].second, es, b,
+                         tile0 + ti, tile0 + tj});
         for (int g = 0; g < groups; ++g) {
           const int work = k * static_cast<int>(groups) + g;
           per_tile[tile0 + ti].push_back(2 * work);
@@ -678,6 +686,8 @@ int ensure_pair_items(pyiid_ctx* c, int nq, bool weighted, int n_elem) {
       }
     tile0 += nt;
   }
+  if (static_cast<int64_t>(v.size()) * groups > INT32_MAX / 2)
+    return fail(PYIID_E_INVALID, "too many atom pairs");
   std::vector<int> entries;
   for (size_t t = 0; t < tiles.size(); ++t) {
     tiles[t].begin = static_cast<int>(entries.size());
@@ -685,6 +695,7 @@ int ensure_pair_items(pyiid_ctx* c, int nq, bool weighted, int n_elem) {
     tiles[t].end = static_cast<int>(entries.size());
   }
   CHK(upload_items(c, s, v, tiles));
+  CHK(upload_perm(c, s, perm));
   CHK(c->pair_entries.reserve(sizeof(int) * std::max<size_t>(1, entries.size())));
   if (!entries.empty())
     HIPCHK(hipMemcpy(c->pair_entries.p, entries.data(), sizeof(int) * entries.size(),
@@ -700,13 +711,13 @@ int launch_contract(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
   const int nq = t.nq;
   const double dq = static_cast<double>(static_cast<float>(qbin));
-  CHK(ensure_pair_items(c, nq, weighted_table(t), t.n_elem));
+  CHK(ensure_pair_items(c, nq, weighted_table(t)));
   const ItemSet& s = c->pair_items;
   CHK(c->pfpart.reserve(sizeof(float) * 384 * static_cast<size_t>(std::max(1, s.n_items)) *
                         c->pf_groups));
-  HIPCHK(launch_pair_force(c->stream, c->pos.as<AtomRec>(), t.tab.as<float>(), t.n_elem, nq, dq,
-                           s.items.as<Item>(), s.n_items, c->pf_groups, c->pf_span,
-                           weighted_table(t), c->u.as<float>(), c->uq.as<float>(),
+  HIPCHK(launch_pair_force(c->stream, c->pos.as<AtomRec>(), s.perm.as<int32_t>(),
+                           t.tab.as<float>(), nq, dq, s.items.as<Item>(), s.n_items, c->pf_groups,
+                           c->pf_span, weighted_table(t), c->u.as<float>(), c->uq.as<float>(),
                            c->pfpart.as<float>()));
   return PYIID_OK;
 }
@@ -823,7 +834,7 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
   // split changes both)
   CHK(c->io_b.reserve(sizeof(float) * static_cast<size_t>(c->n) * 3 * std::max(1, t.nq)));
   auto trial = [&]() {
-    CHK(run_fused(c, slot, qbin, nullptr, false));
+    CHK(run_fused(c, slot, qbin, false));
     HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, c->n, t.nq,
                               t.gs.as<double>(), c->io_b.p, false));
     return PYIID_OK;
@@ -847,34 +858,23 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
   return PYIID_OK;
 }
 
-// forces (n x 3 doubles) = scale * sum_q u[chain][q] grad(i, c, q): the
-// unordered-pair kernel when its tables fit in LDS, else the fused kernel.
+// forces (n x 3 doubles) = scale * sum_q u[chain][q] grad(i, c, q), by the
+// unordered-pair contracted-force kernel.
 int contract_forces(pyiid_ctx* c, int slot, double qbin, double scale, double* out) {
   Table& t = c->tab[slot];
   const int nq = t.nq, B = c->n_chain;
   CHK(ensure_packed(c));
   CHK(ensure_norm(c, slot));
-  const bool pair_ok = !weighted_table(t) ||
-                       sizeof(float) * (static_cast<size_t>(t.n_elem) * nq + 2 * kWaves * 32) <=
-                           kPairForceLds;
-  if (pair_ok) {
-    CHK(c->uq.reserve(sizeof(float) * static_cast<size_t>(std::max(1, nq)) * B));
-    HIPCHK(launch_uq(c->stream, c->u.as<float>(), B, nq,
-                     static_cast<double>(static_cast<float>(qbin)), c->uq.as<float>()));
-    CHK(tune_contract(c, slot, qbin));
-    Timed tm(c, 4);
-    CHK(launch_contract(c, slot, qbin));
-    CHK(tm.done());
-    HIPCHK(launch_reduce_pair_force(c->stream, c->pfpart.as<float>(), c->pair_entries.as<int>(),
-                                    c->pair_items.ranges.as<Range>(), c->pair_items.n_ranges,
-                                    scale, out));
-    return PYIID_OK;
-  }
-  CHK(run_fused(c, slot, qbin, c->u.as<float>()));
-  const int n_qchunk =
-      ((nq + kFusedWaves * c->grad_m - 1) / (kFusedWaves * c->grad_m)) * kFusedWaves;
-  HIPCHK(launch_reduce_force(c->stream, c->forcepart.as<float>(), c->fused_items.ranges.as<Range>(),
-                             c->fused_items.n_ranges, n_qchunk, scale, out));
+  CHK(c->uq.reserve(sizeof(float) * static_cast<size_t>(std::max(1, nq)) * B));
+  HIPCHK(launch_uq(c->stream, c->u.as<float>(), B, nq,
+                   static_cast<double>(static_cast<float>(qbin)), c->uq.as<float>()));
+  CHK(tune_contract(c, slot, qbin));
+  Timed tm(c, 4);
+  CHK(launch_contract(c, slot, qbin));
+  CHK(tm.done());
+  HIPCHK(launch_reduce_pair_force(c->stream, c->pfpart.as<float>(), c->pair_entries.as<int>(),
+                                  c->pair_items.ranges.as<Range>(), c->pair_items.n_ranges,
+                                  c->pair_items.perm.as<int32_t>(), scale, out));
   return PYIID_OK;
 }
 
@@ -1387,7 +1387,7 @@ int pyiid_destroy(pyiid_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   destroy_graph(c->hmc);
   for (Buf* b : {&c->posf, &c->elem, &c->pos, &c->counts_d, &c->natoms_d, &c->tmat, &c->gpart,
-                 &c->fpart, &c->forcepart, &c->fq64, &c->fq32, &c->g64, &c->io_a, &c->io_b,
+                 &c->fpart, &c->fq64, &c->fq32, &c->g64, &c->io_a, &c->io_b,
                  &c->io_c, &c->w, &c->out5, &c->u, &c->gobs, &c->fq_items.items,
                  &c->fq_items.ranges, &c->fq_items.perm, &c->fused_items.items, &c->fused_items.ranges,
                  &c->fused_items.perm, &c->hmc.mass,
@@ -1397,7 +1397,7 @@ int pyiid_destroy(pyiid_ctx* c) {
                  &c->sur.d, &c->sur.cbuf, &c->sur.eatom, &c->sur.fsur, &c->sur.phi,
                  &c->sur.gpart, &c->sur.gout, &c->sur.viol, &c->sur.viol_count,
                  &c->sur.atom_chain, &c->sur.tiles64, &c->sur.tiles256, &c->sur.chunks,
-                 &c->sur.granges, &c->pair_items.items, &c->pair_items.ranges,
+                 &c->sur.granges, &c->pair_items.items, &c->pair_items.ranges, &c->pair_items.perm,
                  &c->pair_entries, &c->uq, &c->pfpart})
     b->release();
   for (Table& t : c->tab) {
@@ -1593,7 +1593,7 @@ int pyiid_grad_fq(pyiid_ctx* c, int slot, double qbin, void* grad, float* fq, in
     }
     return finish(c, flags);
   }
-  CHK(run_fused(c, slot, qbin, nullptr));
+  CHK(run_fused(c, slot, qbin));
   Table& t = c->tab[slot];
   void* gdst = grad;
   if (!(flags & PYIID_DEVICE)) {

@@ -228,18 +228,15 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 //   grid: x = work item (i tile x j range), y = Q group (kFusedWaves waves x M bins)
 //   gpart[split][i][c][q]  float  sum_j t_ij(q) (q_i - q_j)_c / r^2 (ff-weighted)
 //   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
-//   CONTRACT: instead of gpart, force_part[item][qchunk][lane][c] =
-//             sum_q u[chain][q] * gpart(i, c, q)
 // Weighted: the item's j indices are positions in jperm (the atoms in
 // element order) and its j range is of one element, item.pad; the loop is
 // the single-element one and f_i(Q) f_j(Q) scales the lane's sums at the end.
 // ---------------------------------------------------------------------------
-template <int M, bool WEIGHTED, bool CONTRACT>
+template <int M, bool WEIGHTED>
 __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
     const AtomRec* __restrict__ pos, const int32_t* __restrict__ jperm, int n_total,
     const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
-    float* __restrict__ gpart, double* __restrict__ fpart, const float* __restrict__ u,
-    float* __restrict__ force_part) {
+    float* __restrict__ gpart, double* __restrict__ fpart) {
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFusedSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
@@ -368,32 +365,15 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
   const double fmine = lane_reduce_f64<M>(aF, smem, wave, lane);
   if (lane < M && q0 + lane < nq) fpart[(size_t)blockIdx.x * nq + q0 + lane] = fmine;
 
-  if constexpr (CONTRACT) {
-    const float* uc = u + (size_t)it.chain * nq;
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-      const float um = (q0 + m < nq) ? uc[q0 + m] : 0.0f;
-      fx = fmaf(um, ax[m], fx);
-      fy = fmaf(um, ay[m], fy);
-      fz = fmaf(um, az[m], fz);
-    }
-    const size_t p = (size_t)blockIdx.x * (gridDim.y * kFusedWaves) + qchunk;
-    float* dst = force_part + (p * 64 + lane) * 3;
-    dst[0] = fx;
-    dst[1] = fy;
-    dst[2] = fz;
-  } else {
-    // gpart[split][i][c][q0 + m]: rows of the tile are atoms, stride 3 nq floats.
-    const int i0 = it.i_base;
-    const int n_rows = it.i_end - it.i_base;
-    const int n_valid = max(0, min(M, nq - q0));
-    float* base = gpart + (((size_t)it.split * n_total + i0) * 3) * nq + (q0 < nq ? q0 : 0);
-    const size_t stride = (size_t)3 * nq;
-    store_tile<M>(ax, smem, wave, lane, base, stride, n_rows, n_valid);
-    store_tile<M>(ay, smem, wave, lane, base + nq, stride, n_rows, n_valid);
-    store_tile<M>(az, smem, wave, lane, base + 2 * (size_t)nq, stride, n_rows, n_valid);
-  }
+  // gpart[split][i][c][q0 + m]: rows of the tile are atoms, stride 3 nq floats.
+  const int i0 = it.i_base;
+  const int n_rows = it.i_end - it.i_base;
+  const int n_valid = max(0, min(M, nq - q0));
+  float* base = gpart + (((size_t)it.split * n_total + i0) * 3) * nq + (q0 < nq ? q0 : 0);
+  const size_t stride = (size_t)3 * nq;
+  store_tile<M>(ax, smem, wave, lane, base, stride, n_rows, n_valid);
+  store_tile<M>(ay, smem, wave, lane, base + nq, stride, n_rows, n_valid);
+  store_tile<M>(az, smem, wave, lane, base + 2 * (size_t)nq, stride, n_rows, n_valid);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,8 +595,10 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
 }
 
 // ---------------------------------------------------------------------------
-// Contracted forces over unordered pairs.  An item is a pair of 64-atom tiles
-// (I, J), J <= I, of one chain; a wave takes one item and one Q group.  Lane l
+// Contracted forces over unordered pairs.  An item is a pair of tiles (I, J),
+// J <= I, of at most 64 positions of one chain in perm (the atom order of the
+// work list; for a weighted table each tile is of one element); a wave takes
+// one item and one Q group.  Lane l
 // holds atom I_l; the J tile's atoms and their force accumulators turn round
 // the wave one lane per step, so lane l meets J_((l+s) mod 64) at step s and
 // every unordered pair is met once (a diagonal item, I = J, runs steps 1..32
@@ -626,7 +608,8 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
 // and the pair adds S (q_i - q_j) / r^2 to atom i and the negative to atom j
 // with S = A - B / r: the same quantity as sum_q u_q grad(i, c, q) of the
 // fused kernel (t = Q cos - sin / r there), without per-bin accumulators.
-//   part[work][side][64][3]  (side 0: I tile, 1: J tile; work = item * groups + g)
+//   part[work][side][64][3]  (side 0: I tile, 1: J tile; work = item * groups + g,
+//                             rows are tile positions)
 // The recurrences restart every kPfChunk bins from an anchor: an FP64-reduced
 // sincos every kPfAnchor chunks, an FP32 rotation by kPfChunk dtheta between.
 // ---------------------------------------------------------------------------
@@ -635,9 +618,10 @@ constexpr int kPfAnchor = 4;  // chunks per FP64-reduced anchor
 
 template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void pair_force_kernel(
-    const AtomRec* __restrict__ pos, const float* __restrict__ ftab, int n_elem, int nq,
-    double dq, const Item* __restrict__ items, int n_work, int n_groups, int span,
-    const float* __restrict__ u, const float* __restrict__ uq, float* __restrict__ part) {
+    const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
+    const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
+    int n_work, int n_groups, int span, const float* __restrict__ u,
+    const float* __restrict__ uq, float* __restrict__ part) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -647,16 +631,21 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
   const int g = active ? work - item * n_groups : 0;
   const Item it = items[item];
   const int q_lo = g * span;
+  // u and u Q of the wave's Q group; weighted: times f_I(Q) f_J(Q), the
+  // item's tiles being of one element each (item.split = e_I | e_J << 16)
   float2* uw = reinterpret_cast<float2*>(lds) + static_cast<size_t>(wave) * span;
   const float* uc = u + static_cast<size_t>(it.chain) * nq;
   const float* uqc = uq + static_cast<size_t>(it.chain) * nq;
+  const float* fI = ftab + static_cast<size_t>(it.split & 0xffff) * nq;
+  const float* fJ = ftab + static_cast<size_t>(it.split >> 16) * nq;
   for (int k = lane; k < span; k += 64) {
     const int q = q_lo + k;
-    uw[k] = (active && q < nq) ? make_float2(uc[q], uqc[q]) : make_float2(0.f, 0.f);
-  }
-  float* tab = lds + 2 * kWaves * span;
-  if constexpr (WEIGHTED) {
-    for (int t = threadIdx.x; t < n_elem * nq; t += blockDim.x) tab[t] = ftab[t];
+    float2 v = make_float2(0.f, 0.f);
+    if (active && q < nq) {
+      const float ff = WEIGHTED ? fI[q] * fJ[q] : 1.0f;
+      v = make_float2(uc[q] * ff, uqc[q] * ff);
+    }
+    uw[k] = v;
   }
   __syncthreads();
   if (!active) return;
@@ -664,9 +653,9 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
   const bool diag = it.jbeg == it.i_base;
   const int i = it.i_base + lane;
   const bool i_ok = i < it.i_end;
-  const AtomRec pi = pos[i_ok ? i : it.i_base];
-  int jl = lane;  // slot (atom jbeg + jl) whose data and accumulators this lane holds
-  AtomRec pj = pos[it.jbeg + lane < it.jend ? it.jbeg + lane : it.jbeg];
+  const AtomRec pi = pos[perm[i_ok ? i : it.i_base]];
+  int jl = lane;  // slot (position jbeg + jl) whose data and accumulators this lane holds
+  AtomRec pj = pos[perm[it.jbeg + lane < it.jend ? it.jbeg + lane : it.jbeg]];
   float fix = 0.f, fiy = 0.f, fiz = 0.f, fjx = 0.f, fjy = 0.f, fjz = 0.f;
   const int src = (lane + 1) & 63;
   auto turn = [&]() {
@@ -681,8 +670,6 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
   };
   const int s_begin = diag ? 1 : 0, s_end = diag ? 33 : 64;
   if (diag) turn();
-  const float* ti = tab;
-  if constexpr (WEIGHTED) ti = tab + static_cast<int>(pi.w) * nq;
   for (int s = s_begin; s < s_end; ++s) {
     const bool j_ok = it.jbeg + jl < it.jend;
     const bool keep = i_ok && j_ok && !(diag && s == 32 && lane >= 32);
@@ -691,8 +678,6 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     float s1, k2, lo;
     two_cos(dth, k2, lo, s1);
     const float c1 = 0.5f * k2, lo4 = 4.0f * lo;
-    const float* tj = tab;
-    if constexpr (WEIGHTED) tj = tab + static_cast<int>(pj.w) * nq;
     float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
     // anchors: reduced in FP64 every kPfAnchor chunks, rotated in FP32 by
     // kPfChunk dtheta in between (at most kPfAnchor - 1 rotations)
@@ -716,13 +701,7 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 #pragma unroll
       for (int m = 0; m < kPfChunk; ++m) {
         const float2 w = uw[c0 + m];
-        float wu = w.x, wq = w.y;
-        if constexpr (WEIGHTED) {
-          const int q = min(q_lo + c0 + m, nq - 1);
-          const float ff = ti[q] * tj[q];
-          wu *= ff;
-          wq *= ff;
-        }
+        const float wu = w.x, wq = w.y;
         if (m & 1) {
           a1 = fmaf(wq, cs, a1);
           b1 = fmaf(wu, sn, b1);
@@ -765,7 +744,8 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
 __global__ void reduce_pair_force_kernel(const float* __restrict__ part,
                                          const int* __restrict__ entries,
                                          const Range* __restrict__ tiles, int n_tiles,
-                                         double conv, double* __restrict__ out) {
+                                         const int32_t* __restrict__ perm, double conv,
+                                         double* __restrict__ out) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= (int64_t)n_tiles * 192) return;
   const int tile = static_cast<int>(t / 192), rem = static_cast<int>(t % 192);
@@ -774,7 +754,7 @@ __global__ void reduce_pair_force_kernel(const float* __restrict__ part,
   double acc = 0.0;
   for (int e = tr.begin; e < tr.end; ++e)
     acc += static_cast<double>(part[static_cast<size_t>(entries[e]) * 192 + rem]);
-  out[static_cast<size_t>(tr.i_base) * 3 + rem] = acc * conv;
+  out[static_cast<size_t>(perm[tr.i_base + rem / 3]) * 3 + rem % 3] = acc * conv;
 }
 
 // uq[b][q] = u[b][q] * Q_q, Q_q = float32(q * dq) as in the Debye kernels
@@ -852,25 +832,6 @@ __global__ void reduce_grad_kernel(const float* __restrict__ gpart, int n_split,
   }
 }
 
-// force[i][c] = conv * sum over the tile's items and Q chunks (FP64); one
-// thread per (tile, lane, component).
-__global__ void reduce_force_kernel(const float* __restrict__ fpart, const Range* __restrict__ tiles,
-                                    int n_tiles, int n_qchunk, double conv,
-                                    double* __restrict__ out) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n_tiles * 192) return;
-  const int tile = static_cast<int>(t / 192), rem = static_cast<int>(t % 192);
-  const int lane = rem / 3, c = rem % 3;
-  const Range tr = tiles[tile];
-  const int i = tr.i_base + lane;
-  if (i >= tr.i_end) return;
-  double acc = 0.0;
-  for (int p = tr.begin; p < tr.end; ++p)
-    for (int qc = 0; qc < n_qchunk; ++qc)
-      acc += static_cast<double>(fpart[(((size_t)p * n_qchunk + qc) * 64 + lane) * 3 + c]);
-  out[(size_t)i * 3 + c] = acc * conv;
-}
-
 // na(q) = N * mean_{i>j} f_i f_j = ((sum_e n_e f_e)^2 - sum_e n_e f_e^2) / (N - 1),
 // per chain (counts[chain][e], n_atoms[chain]).  Unweighted kernels (one
 // element) accumulate without f_i f_j = f^2, so their scales carry w = f^2;
@@ -938,68 +899,56 @@ int grid_for(int64_t total, int block) {
 template <int M>
 hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, const int32_t* jperm, int n_total,
                           const float* ftab, int nq, double dq, const Item* items, int n_items,
-                          bool weighted, float* gpart, double* fpart, const float* u,
-                          float* force_part) {
+                          bool weighted, float* gpart, double* fpart) {
   dim3 grid(n_items, (nq + kFusedWaves * M - 1) / (kFusedWaves * M));
   dim3 block(64 * kFusedWaves);
-  const bool contract = (u != nullptr);
-  if (weighted) {
-    if (contract)
-      hipLaunchKernelGGL((fused_grad_kernel<M, true, true>), grid, block, 0, st, pos, jperm,
-                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
-    else
-      hipLaunchKernelGGL((fused_grad_kernel<M, true, false>), grid, block, 0, st, pos, jperm,
-                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
-  } else {
-    if (contract)
-      hipLaunchKernelGGL((fused_grad_kernel<M, false, true>), grid, block, 0, st, pos, jperm,
-                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
-    else
-      hipLaunchKernelGGL((fused_grad_kernel<M, false, false>), grid, block, 0, st, pos, jperm,
-                         n_total, ftab, nq, dq, items, gpart, fpart, u, force_part);
-  }
+  if (weighted)
+    hipLaunchKernelGGL((fused_grad_kernel<M, true>), grid, block, 0, st, pos, jperm, n_total, ftab,
+                       nq, dq, items, gpart, fpart);
+  else
+    hipLaunchKernelGGL((fused_grad_kernel<M, false>), grid, block, 0, st, pos, jperm, n_total,
+                       ftab, nq, dq, items, gpart, fpart);
   return hipGetLastError();
 }
 
 hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
                              int n_total, const float* ftab, int nq, double dq, const Item* items,
-                             int n_items, bool weighted, float* gpart, double* fpart,
-                             const float* u, float* force_part) {
+                             int n_items, bool weighted, float* gpart, double* fpart) {
   if (n_items <= 0) return hipSuccess;
   if (m == 16)
     return launch_fused_m<16>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items, weighted,
-                              gpart, fpart, u, force_part);
+                              gpart, fpart);
   if (m == 32)
     return launch_fused_m<32>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items, weighted,
-                              gpart, fpart, u, force_part);
+                              gpart, fpart);
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_pair_force(hipStream_t st, const AtomRec* pos, const float* ftab, int n_elem,
-                             int nq, double dq, const Item* items, int n_items, int n_groups,
-                             int span, bool weighted, const float* u, const float* uq,
-                             float* part) {
+hipError_t launch_pair_force(hipStream_t st, const AtomRec* pos, const int32_t* perm,
+                             const float* ftab, int nq, double dq, const Item* items, int n_items,
+                             int n_groups, int span, bool weighted, const float* u,
+                             const float* uq, float* part) {
   const int n_work = n_items * n_groups;
   if (n_work <= 0) return hipSuccess;
-  const size_t lds = sizeof(float) * (2 * kWaves * static_cast<size_t>(span) +
-                                      (weighted ? static_cast<size_t>(n_elem) * nq : 0));
+  const size_t lds = sizeof(float) * 2 * kWaves * static_cast<size_t>(span);
   if (span % kPfChunk != 0 || lds > kPairForceLds) return hipErrorInvalidValue;
   dim3 grid((n_work + kWaves - 1) / kWaves);
   if (weighted)
-    hipLaunchKernelGGL(pair_force_kernel<true>, grid, dim3(256), lds, st, pos, ftab, n_elem, nq,
-                       dq, items, n_work, n_groups, span, u, uq, part);
+    hipLaunchKernelGGL(pair_force_kernel<true>, grid, dim3(256), lds, st, pos, perm, ftab, nq, dq,
+                       items, n_work, n_groups, span, u, uq, part);
   else
-    hipLaunchKernelGGL(pair_force_kernel<false>, grid, dim3(256), lds, st, pos, ftab, n_elem, nq,
+    hipLaunchKernelGGL(pair_force_kernel<false>, grid, dim3(256), lds, st, pos, perm, ftab, nq,
                        dq, items, n_work, n_groups, span, u, uq, part);
   return hipGetLastError();
 }
 
 hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int* entries,
-                                    const Range* tiles, int n_tiles, double conv, double* out) {
+                                    const Range* tiles, int n_tiles, const int32_t* perm,
+                                    double conv, double* out) {
   const int64_t total = (int64_t)n_tiles * 192;
   if (total <= 0) return hipSuccess;
   hipLaunchKernelGGL(reduce_pair_force_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
-                     dim3(256), 0, st, part, entries, tiles, n_tiles, conv, out);
+                     dim3(256), 0, st, part, entries, tiles, n_tiles, perm, conv, out);
   return hipGetLastError();
 }
 
@@ -1089,15 +1038,6 @@ hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, i
   else
     hipLaunchKernelGGL(reduce_grad_kernel<float>, grid, dim3(256), 0, st, gpart, n_split, rows,
                        nq, gscale, static_cast<float*>(out));
-  return hipGetLastError();
-}
-
-hipError_t launch_reduce_force(hipStream_t st, const float* force_part, const Range* tiles,
-                               int n_tiles, int n_qchunk, double conv, double* out) {
-  const int64_t total = (int64_t)n_tiles * 192;
-  if (total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(reduce_force_kernel, dim3(static_cast<unsigned>((total + 255) / 256)),
-                     dim3(256), 0, st, force_part, tiles, n_tiles, n_qchunk, conv, out);
   return hipGetLastError();
 }
 

@@ -36,29 +36,30 @@ struct Range {
 };
 
 // ---- launch wrappers (debye.hip) ----
-// Ordered-pair F + gradient, m = 16 or 32 Q bins per wave.  Non-contract:
-// gpart[item.split][i][c][q] (float) for i in the item's tile.  Contract
-// (u != null): force_part[item][qchunk][64][3] = sum_q u[chain][q] * grad(i, c, q),
-// qchunk over ceil(nq / (kFusedWaves m)) * kFusedWaves.  fpart[item][q]: F partial (double).
-// Weighted: the j indices of an item are positions in jperm and item.pad is
-// the element of the item's j range; unweighted: atom indices.
+// Ordered-pair F + gradient, m = 16 or 32 Q bins per wave:
+// gpart[item.split][i][c][q] (float) for i in the item's tile and
+// fpart[item][q] (F partial, double).  Weighted: the j indices of an item are
+// positions in jperm and item.pad is the element of the item's j range;
+// unweighted: atom indices.
 hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
                              int n_total, const float* ftab, int nq, double dq, const Item* items,
-                             int n_items, bool weighted, float* gpart, double* fpart,
-                             const float* u, float* force_part);
-// Contracted forces over unordered pairs: items are 64-atom tile pairs (I, J),
-// J <= I, of one chain (Item.tile = I's and Item.pad = J's chain-global tile
-// index); n_groups Q groups of `span` bins (a multiple of 32) each.
-// part[(item * n_groups + g)][side][64][3], side 0 = I, 1 = J.
+                             int n_items, bool weighted, float* gpart, double* fpart);
+// Contracted forces over unordered pairs: items are tile pairs (I, J), J <= I,
+// of at most 64 positions of one chain in perm (Item.tile = I's and Item.pad =
+// J's chain-global tile index; weighted: Item.split = e_I | e_J << 16, each
+// tile of one element); n_groups Q groups of `span` bins (a multiple of 32)
+// each.  part[(item * n_groups + g)][side][64][3], side 0 = I, 1 = J.
 constexpr size_t kPairForceLds = 64 * 1024;
-hipError_t launch_pair_force(hipStream_t st, const AtomRec* pos, const float* ftab, int n_elem,
-                             int nq, double dq, const Item* items, int n_items, int n_groups,
-                             int span, bool weighted, const float* u, const float* uq,
-                             float* part);
-// force[i][c] = conv * sum_{e in entries[tile]} part[e][lane][c] (FP64), where
-// an entry is work * 2 + side; tiles[t] = {entry begin, end, i_base, i_end}.
+hipError_t launch_pair_force(hipStream_t st, const AtomRec* pos, const int32_t* perm,
+                             const float* ftab, int nq, double dq, const Item* items, int n_items,
+                             int n_groups, int span, bool weighted, const float* u,
+                             const float* uq, float* part);
+// force[perm[p]][c] = conv * sum_{e in entries[tile]} part[e][row][c] (FP64)
+// for the tile's positions p = i_base + row, where an entry is work * 2 +
+// side; tiles[t] = {entry begin, end, i_base, i_end}.
 hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int* entries,
-                                    const Range* tiles, int n_tiles, double conv, double* out);
+                                    const Range* tiles, int n_tiles, const int32_t* perm,
+                                    double conv, double* out);
 // uq[b][q] = u[b][q] * float32(q dq)
 hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
 // Unordered-pair F only: fpart[item][q] (double), m Q bins per wave (64, 72,
@@ -75,9 +76,6 @@ hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range*
 // grad[i][c][q] = gscale[q] * sum_s gpart[s][i][c][q]
 hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, int64_t n,
                               int nq, const double* gscale, void* out, bool out_f64);
-// force[i][c] = conv * sum over the tile's items and Q chunks of force_part
-hipError_t launch_reduce_force(hipStream_t st, const float* force_part, const Range* tiles,
-                               int n_tiles, int n_qchunk, double conv, double* out);
 // per-chain normalisation: na, 2w/na, w/na, -w/na  ([chain][q])
 hipError_t launch_norm(hipStream_t st, const float* ftab, const int64_t* counts, int n_elem,
                        int nq, const int64_t* n_atoms, int n_chain, bool weighted, double* na,

@@ -170,10 +170,10 @@ def test_steps_near_multiples_of_pi(ctx, k, off):
 
 @pytest.mark.parametrize("qbin,nq", [(0.03, 834), (0.003, 8334)])
 def test_contracted_forces_both_kernels(ctx, qbin, nq):
-    """F(Q)-Rw energy and forces of a Pt/Ni alloy.  At 834 bins the element
-    table fits the unordered-pair force kernel's LDS; at 8,334 bins
-    (2 x 8,334 floats > 64 KB) the call takes the fused kernel's contracted
-    variant instead.  Both against the oracle."""
+    """F(Q)-Rw energy and forces of a Pt/Ni alloy through the unordered-pair
+    contracted-force kernel: at 834 bins in one Q group, at 8,334 bins in
+    several (a wave's u and u Q in LDS hold at most 2,048 bins).  Both
+    against the oracle."""
     base = structures.fcc_nearest(43, a=3.724)
     pos = structures.jitter(base, 0.05, seed=4).astype(np.float32)
     numbers = structures.random_alloy(len(pos), 78, 28, seed=6)
@@ -266,11 +266,10 @@ def test_fixed_launch_shapes_repeat_bit_for_bit():
 
 def test_ten_elements_and_mixed_batch(ctx):
     """All ten tabulated elements in one particle, in random order: F, the
-    FP64 gradient and the F(Q)-Rw forces (10 rows x 1,700 bins exceed the
-    unordered-pair force kernel's LDS, so the fused kernel's contracted
-    variant runs), then a batch of chains of different compositions (one
-    element, two, ten, a single atom) through the F-only kernel, each chain
-    against the oracle."""
+    FP64 gradient and the F(Q)-Rw forces (tiles of one element each, up to
+    ten tile pairs' worth of element weights), then a batch of chains of
+    different compositions (one element, two, ten, a single atom) through
+    the F-only kernel, each chain against the oracle."""
     zs = np.array([6, 8, 14, 26, 28, 29, 46, 47, 78, 79])
     rs = np.random.RandomState(11)
     base = structures.fcc_nearest(201, a=3.9)
