@@ -280,6 +280,19 @@ def kernel_suite(reps=3):
     out["surrogate_forces_call"] = {"ms": (time.perf_counter() - t0) / reps * 1e3,
                                     "atoms": nat}
     h.close()
+    # the same chains as 50/50 Pt/Ni random alloys (weighted contracted forces)
+    alloy = [structures.random_alloy(len(x), 78, 28, seed=k) for k, x in enumerate(starts)]
+    gta = workloads.target_pdfs(scat, targs, alloy)
+    ta, ia, _ = element_table(np.concatenate(alloy), scat.pdf_qbin, nqp)
+    cpa = _lib.Context(0)
+    cpa.set_batch(starts)
+    cpa.set_elements(ia)
+    cpa.set_scatter(_lib.SLOT_PDF, ta)
+    cpa.set_transform(scat.exp["rstep"], scat.pdf_qbin, rgrid, scat.exp["qmin"], nqp)
+    ms = timed(cpa, _lib.KERNEL_PAIR_FORCE, lambda: cpa.pdf_energy_forces(
+        _lib.SLOT_PDF, scat.pdf_qbin, gta, _lib.POT_RW, 100.0, nat))
+    out["pair_force_kernel_alloy"] = {"ms": ms, "pair_q_per_s": kpairs * nqp / (ms * 1e-3)}
+    cpa.close()
     return out
 
 
