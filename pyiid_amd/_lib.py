@@ -140,7 +140,11 @@ def _ptr(a):
 
 class Context:
     """One device context: a HIP stream plus the device-resident structure,
-    scatter tables, transform and workspaces (``pyiid_ctx``)."""
+    scatter tables, transform and workspaces (``pyiid_ctx``).
+
+    Launch shapes are autotuned per problem shape; the environment variable
+    PYIID_AUTOTUNE=0 starts every context with fixed shapes instead
+    (results that repeat bit for bit), as ``set_autotune(False)`` does."""
 
     def __init__(self, device=0):
         self.lib = load()
@@ -148,6 +152,8 @@ class Context:
         check(self.lib.pyiid_create(int(device), ctypes.byref(h)))
         self.handle = h
         self.device = device
+        if os.environ.get("PYIID_AUTOTUNE", "1") == "0":
+            self.set_autotune(False)
 
     def close(self):
         if getattr(self, "handle", None):

@@ -264,6 +264,29 @@ def test_fixed_launch_shapes_repeat_bit_for_bit():
     assert abs(t[2] - a[2]) < 1e-6 and rel(t[3], a[3]) < 1e-5
 
 
+def test_environment_switch_fixes_launch_shapes(monkeypatch):
+    """PYIID_AUTOTUNE=0 gives every new context the fixed shapes: F equals,
+    bit for bit, that of a context switched off by set_autotune(False)."""
+    pos = structures.jitter(structures.fcc_nearest(180, a=4.08), 0.05, seed=31).astype(np.float32)
+    numbers = np.full(len(pos), 79)
+
+    def fq(env, switch):
+        if env:
+            monkeypatch.setenv("PYIID_AUTOTUNE", "0")
+        else:
+            monkeypatch.delenv("PYIID_AUTOTUNE", raising=False)
+        c = _lib.Context(0)
+        try:
+            if switch:
+                c.set_autotune(False)
+            load(c, pos, numbers, 0.05, 600)
+            return c.fq(_lib.SLOT_FQ, 0.05, 600)
+        finally:
+            c.close()
+
+    np.testing.assert_array_equal(fq(True, False), fq(False, True))
+
+
 def test_ten_elements_and_mixed_batch(ctx):
     """All ten tabulated elements in one particle, in random order: F, the
     FP64 gradient and the F(Q)-Rw forces (tiles of one element each, up to
