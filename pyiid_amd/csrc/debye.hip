@@ -616,6 +616,81 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
 constexpr int kPfChunk = 32;
 constexpr int kPfAnchor = 4;  // chunks per FP64-reduced anchor
 
+// S = A - B / r of NP pairs (geometry g[p]) over the wave's `span` bins from
+// q_lo, with uw[k] = (u, u Q) of bin q_lo + k:
+//   A = sum u Q cos(Q r),  B = sum u sin(Q r).
+// sin and cos follow the three-term recurrence (one FMA each per bin) with
+// 2 cos(dtheta) = k2 + lo, the low part added every fourth bin: exact
+// frequency on average, bounded error for every dtheta (close pairs and steps
+// near multiples of pi included), no per-wave branch.  The recurrences restart
+// every kPfChunk bins from an anchor: reduced in FP64 every kPfAnchor chunks,
+// rotated in FP32 by kPfChunk dtheta in between.  The NP pairs share each
+// bin's u reads.
+template <int NP>
+__device__ __forceinline__ void pf_sums(const float2* uw, int span, int q_lo, double dq,
+                                        const PairGeom (&g)[NP], float (&S)[NP]) {
+  double dth[NP];
+  float k2[NP], lo4[NP], c1[NP], s1[NP], rs[NP], rc[NP], an_s[NP], an_c[NP];
+  float a0[NP], a1[NP], b0[NP], b1[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    dth[p] = dq * g[p].r;
+    float lo;
+    two_cos(dth[p], k2[p], lo, s1[p]);
+    c1[p] = 0.5f * k2[p];
+    lo4[p] = 4.0f * lo;
+    rs[p] = 0.f;
+    rc[p] = 1.f;
+    if (span > kPfChunk) sincos_red(static_cast<double>(kPfChunk) * dth[p], rs[p], rc[p]);
+    an_s[p] = 0.f;
+    an_c[p] = 1.f;
+    a0[p] = a1[p] = b0[p] = b1[p] = 0.f;
+  }
+  for (int c0 = 0; c0 < span; c0 += kPfChunk) {
+    float sn[NP], cs[NP], sp[NP], cp[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      if ((c0 / kPfChunk) % kPfAnchor == 0) {
+        sincos_red(static_cast<double>(q_lo + c0) * dth[p], an_s[p], an_c[p]);
+      } else {
+        const float s2 = fmaf(an_s[p], rc[p], an_c[p] * rs[p]);
+        an_c[p] = fmaf(an_c[p], rc[p], -an_s[p] * rs[p]);
+        an_s[p] = s2;
+      }
+      sn[p] = an_s[p];
+      cs[p] = an_c[p];
+      sp[p] = fmaf(sn[p], c1[p], -cs[p] * s1[p]);
+      cp[p] = fmaf(cs[p], c1[p], sn[p] * s1[p]);
+    }
+#pragma unroll
+    for (int m = 0; m < kPfChunk; ++m) {
+      const float2 w = uw[c0 + m];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        if (m & 1) {
+          a1[p] = fmaf(w.y, cs[p], a1[p]);
+          b1[p] = fmaf(w.x, sn[p], b1[p]);
+        } else {
+          a0[p] = fmaf(w.y, cs[p], a0[p]);
+          b0[p] = fmaf(w.x, sn[p], b0[p]);
+        }
+        float sn2 = fmaf(k2[p], sn[p], -sp[p]);
+        float cs2 = fmaf(k2[p], cs[p], -cp[p]);
+        if ((m & 3) == 3) {
+          sn2 = fmaf(lo4[p], sn[p], sn2);
+          cs2 = fmaf(lo4[p], cs[p], cs2);
+        }
+        sp[p] = sn[p];
+        cp[p] = cs[p];
+        sn[p] = sn2;
+        cs[p] = cs2;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) S[p] = (a0[p] + a1[p]) - (b0[p] + b1[p]) * g[p].inv_r;
+}
+
 template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void pair_force_kernel(
     const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
@@ -662,73 +737,51 @@ __global__ __launch_bounds__(256) void pair_force_kernel(
     pj.x = __shfl(pj.x, src, 64);
     pj.y = __shfl(pj.y, src, 64);
     pj.z = __shfl(pj.z, src, 64);
-    pj.w = __shfl(pj.w, src, 64);
     jl = __shfl(jl, src, 64);
     fjx = __shfl(fjx, src, 64);
     fjy = __shfl(fjy, src, 64);
     fjz = __shfl(fjz, src, 64);
   };
-  const int s_begin = diag ? 1 : 0, s_end = diag ? 33 : 64;
-  if (diag) turn();
-  for (int s = s_begin; s < s_end; ++s) {
-    const bool j_ok = it.jbeg + jl < it.jend;
-    const bool keep = i_ok && j_ok && !(diag && s == 32 && lane >= 32);
-    const PairGeom pg = pair_geom(pi, pj, keep);
-    const double dth = dq * pg.r;
-    float s1, k2, lo;
-    two_cos(dth, k2, lo, s1);
-    const float c1 = 0.5f * k2, lo4 = 4.0f * lo;
-    float a0 = 0.f, a1 = 0.f, b0 = 0.f, b1 = 0.f;
-    // anchors: reduced in FP64 every kPfAnchor chunks, rotated in FP32 by
-    // kPfChunk dtheta in between (at most kPfAnchor - 1 rotations)
-    float rs = 0.f, rc = 1.f;
-    if (span > kPfChunk) sincos_red(static_cast<double>(kPfChunk) * dth, rs, rc);
-    float an_s = 0.f, an_c = 1.f;
-    for (int c0 = 0; c0 < span; c0 += kPfChunk) {
-      if ((c0 / kPfChunk) % kPfAnchor == 0) {
-        sincos_red(static_cast<double>(q_lo + c0) * dth, an_s, an_c);
-      } else {
-        const float s2 = fmaf(an_s, rc, an_c * rs);
-        an_c = fmaf(an_c, rc, -an_s * rs);
-        an_s = s2;
-      }
-      float sn = an_s, cs = an_c;
-      // sin and cos follow the three-term recurrence (one FMA each per bin)
-      // with 2 cos(dtheta) = k2 + lo, the low part added every fourth bin:
-      // exact frequency on average, bounded error for every dtheta (close
-      // pairs and steps near multiples of pi included), no per-wave branch.
-      float sp = fmaf(sn, c1, -cs * s1), cp = fmaf(cs, c1, sn * s1);
-#pragma unroll
-      for (int m = 0; m < kPfChunk; ++m) {
-        const float2 w = uw[c0 + m];
-        const float wu = w.x, wq = w.y;
-        if (m & 1) {
-          a1 = fmaf(wq, cs, a1);
-          b1 = fmaf(wu, sn, b1);
-        } else {
-          a0 = fmaf(wq, cs, a0);
-          b0 = fmaf(wu, sn, b0);
-        }
-        float sn2 = fmaf(k2, sn, -sp);
-        float cs2 = fmaf(k2, cs, -cp);
-        if ((m & 3) == 3) {
-          sn2 = fmaf(lo4, sn, sn2);
-          cs2 = fmaf(lo4, cs, cs2);
-        }
-        sp = sn;
-        cp = cs;
-        sn = sn2;
-        cs = cs2;
-      }
-    }
-    const float S = (a0 + a1) - (b0 + b1) * pg.inv_r;
-    fix = fmaf(S, pg.dpx, fix);
-    fiy = fmaf(S, pg.dpy, fiy);
-    fiz = fmaf(S, pg.dpz, fiz);
-    fjx = fmaf(-S, pg.dpx, fjx);
-    fjy = fmaf(-S, pg.dpy, fjy);
-    fjz = fmaf(-S, pg.dpz, fjz);
+  if (diag) {
+    // I = J: offsets 1..32 cover every unordered pair once (the upper half of
+    // offset 32 repeats the lower half and is dropped)
     turn();
+    for (int s = 1; s <= 32; ++s) {
+      const bool keep = i_ok && it.jbeg + jl < it.jend && !(s == 32 && lane >= 32);
+      const PairGeom pg = pair_geom(pi, pj, keep);
+      float S[1];
+      pf_sums<1>(uw, span, q_lo, dq, {pg}, S);
+      fix = fmaf(S[0], pg.dpx, fix);
+      fiy = fmaf(S[0], pg.dpy, fiy);
+      fiz = fmaf(S[0], pg.dpz, fiz);
+      fjx = fmaf(-S[0], pg.dpx, fjx);
+      fjy = fmaf(-S[0], pg.dpy, fjy);
+      fjz = fmaf(-S[0], pg.dpz, fjz);
+      turn();
+    }
+  } else {
+    // two pairs per lane and step, sharing the step's u reads from LDS: the
+    // lane's J slot (offset s) and the slot of lane ^ 32 (offset s + 32);
+    // 32 steps cover the 64 offsets.  The second pair's J-side force goes back
+    // to the lane that holds that slot.
+    for (int s = 0; s < 32; ++s) {
+      AtomRec pk = pj;
+      pk.x = __shfl_xor(pj.x, 32, 64);
+      pk.y = __shfl_xor(pj.y, 32, 64);
+      pk.z = __shfl_xor(pj.z, 32, 64);
+      const int kl = __shfl_xor(jl, 32, 64);
+      const PairGeom g1 = pair_geom(pi, pj, i_ok && it.jbeg + jl < it.jend);
+      const PairGeom g2 = pair_geom(pi, pk, i_ok && it.jbeg + kl < it.jend);
+      float S[2];
+      pf_sums<2>(uw, span, q_lo, dq, {g1, g2}, S);
+      fix = fmaf(S[0], g1.dpx, fmaf(S[1], g2.dpx, fix));
+      fiy = fmaf(S[0], g1.dpy, fmaf(S[1], g2.dpy, fiy));
+      fiz = fmaf(S[0], g1.dpz, fmaf(S[1], g2.dpz, fiz));
+      fjx = fmaf(-S[0], g1.dpx, fjx) + __shfl_xor(-S[1] * g2.dpx, 32, 64);
+      fjy = fmaf(-S[0], g1.dpy, fjy) + __shfl_xor(-S[1] * g2.dpy, 32, 64);
+      fjz = fmaf(-S[0], g1.dpz, fjz) + __shfl_xor(-S[1] * g2.dpz, 32, 64);
+      turn();
+    }
   }
   float* o = part + static_cast<size_t>(work) * 384;
   o[lane * 3 + 0] = fix;
