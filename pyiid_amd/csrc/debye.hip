@@ -306,12 +306,22 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
       }
       __syncthreads();
       if (q0 < nq) {  // wave-uniform
-        for (int k = 0; k < kk; ++k) {
+        // Two register sets: the next atom's setup is read from LDS while the
+        // current atom's bins run.  With two waves per SIMD a wave stalled on
+        // LDS leaves the SIMD single-issuing; the scheduling barriers keep the
+        // reads ahead of the bins (about 3% faster than reading at first use).
+        float ga[9], gb[9];
+        auto load = [&](float (&g)[9], int k) {
           const int o = k * 64 + lane;
-          const float inv_r = geo_f[o], dpx = geo_f[o + kGS], dpy = geo_f[o + 2 * kGS],
-                      dpz = geo_f[o + 3 * kGS], s1 = geo_f[o + 4 * kGS],
-                      k2 = geo_f[o + 5 * kGS], lo4 = geo_f[o + 6 * kGS];
-          float sn = geo_f[o + (7 + 2 * wave) * kGS], cs = geo_f[o + (8 + 2 * wave) * kGS];
+#pragma unroll
+          for (int f = 0; f < 7; ++f) g[f] = geo_f[o + f * kGS];
+          g[7] = geo_f[o + (7 + 2 * wave) * kGS];
+          g[8] = geo_f[o + (8 + 2 * wave) * kGS];
+        };
+        auto bins = [&](const float (&g)[9]) {
+          const float inv_r = g[0], dpx = g[1], dpy = g[2], dpz = g[3], s1 = g[4], k2 = g[5],
+                      lo4 = g[6];
+          float sn = g[7], cs = g[8];
           // Along Q, w = sin(Q r) / r and cs = cos(Q r) follow the three-term
           // recurrence y' = 2 cos(dtheta) y - y_prev (one FMA each per bin)
           // with 2 cos(dtheta) = k2 + lo: the low part is added every fourth
@@ -340,7 +350,18 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
             w = wn;
             cs = cn;
           }
+        };
+        load(ga, 0);
+        int k = 0;
+        for (; k + 1 < kk; k += 2) {
+          load(gb, k + 1);
+          __builtin_amdgcn_sched_barrier(0);
+          bins(ga);
+          load(ga, k + 2 < kk ? k + 2 : k + 1);
+          __builtin_amdgcn_sched_barrier(0);
+          bins(gb);
         }
+        if (k < kk) bins(ga);
       }
       __syncthreads();
     }

@@ -18,3 +18,13 @@ def ctx():
     c = _lib.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture
+def fixed_shapes(monkeypatch):
+    """New contexts skip the timing-based autotuner and use the fixed launch
+    shapes.  Tests that compare runs of two contexts bit for bit need it: the
+    tuners choose by measured time, and two contexts (or a run that starts
+    under graph capture, which keeps the default shape) can pick different
+    bins per wave, which moves the recurrences' anchors and the rounding."""
+    monkeypatch.setenv("PYIID_AUTOTUNE", "0")

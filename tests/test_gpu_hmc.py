@@ -158,7 +158,7 @@ def test_one_proposal_matches_cpu_replay(use_graph):
         np.testing.assert_allclose(got[b], want_pos[b], atol=2e-5)
 
 
-def test_graph_replay_equals_direct_and_lowers_rw():
+def test_graph_replay_equals_direct_and_lowers_rw(fixed_shapes):
     e, pq = S.make_experiment(dict(EXP))
     scat = ElasticScatter(dict(EXP), seed=0)
     chains = make_chains([30, 55, 20], seed=4)

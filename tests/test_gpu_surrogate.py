@@ -115,7 +115,7 @@ def test_training_matches_ridge_fit_on_exact_forces():
     hmc.close()
 
 
-def test_exact_fallback_equals_exact_hmc():
+def test_exact_fallback_equals_exact_hmc(fixed_shapes):
     chains = _chains([25, 18], seed=5)
     runs = []
     for kind in ("exact", "fallback"):
@@ -154,7 +154,7 @@ def test_surrogate_run_energies_are_exact(use_graph):
     hmc.close()
 
 
-def test_graph_and_direct_surrogate_runs_agree():
+def test_graph_and_direct_surrogate_runs_agree(fixed_shapes):
     chains = _chains([26, 33], seed=7)
     res = []
     for g in (False, True):
