@@ -74,6 +74,12 @@ int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launc
  * grouped, so F agrees between them to FP32 rounding.  No counterpart in the
  * reference (its kernels have no launch-shape choice). */
 int pyiid_set_fq_bins_per_wave(pyiid_ctx *ctx, int m);
+/* Q bins per wave of the fused F + gradient kernel: 16 or 32 fixes the launch
+ * shape (with the default j split); 0 (the default) lets the autotuner time
+ * both, with three split sizes each, once per problem shape.  The shapes differ
+ * only in how the bins are grouped, so the results agree to FP32 rounding.
+ * No counterpart in the reference (its kernels have no launch-shape choice). */
+int pyiid_set_grad_bins_per_wave(pyiid_ctx *ctx, int m);
 /* Autotuning of the Debye kernels' launch shapes (fused kernel bins per wave
  * and j split, F-only bins per wave, contracted-force Q groups), on by
  * default: the first call for a problem shape times the candidates and keeps

@@ -49,6 +49,7 @@ _SIGNATURES = [
     ("pyiid_enable_timing", [_P, _I]),
     ("pyiid_kernel_time", [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
     ("pyiid_set_fq_bins_per_wave", [_P, _I]),
+    ("pyiid_set_grad_bins_per_wave", [_P, _I]),
     ("pyiid_set_autotune", [_P, _I]),
     ("pyiid_set_positions", [_P, _P, _I64, _I]),
     ("pyiid_set_elements", [_P, _P, _I64, _I]),
@@ -184,6 +185,10 @@ class Context:
     def set_fq_bins_per_wave(self, m):
         """Fix the F-only kernel's Q bins per wave (64, 72, 80, 88, 96, 112), or 0 to autotune."""
         check(self.lib.pyiid_set_fq_bins_per_wave(self.handle, int(m)))
+
+    def set_grad_bins_per_wave(self, m):
+        """Fix the fused F + gradient kernel's Q bins per wave (16, 32), or 0 to autotune."""
+        check(self.lib.pyiid_set_grad_bins_per_wave(self.handle, int(m)))
 
     def kernel_time(self, kind):
         ms = ctypes.c_double()

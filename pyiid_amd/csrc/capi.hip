@@ -181,6 +181,7 @@ struct pyiid_ctx {
   int grad_m = kGradM;                     // Q bins per wave of the fused kernel
   int fq_m = kFqM;                         // Q bins per wave of the F-only kernel
   int fq_m_fixed = 0;                      // set by pyiid_set_fq_bins_per_wave (0: autotuned)
+  int grad_m_fixed = 0;                    // set by pyiid_set_grad_bins_per_wave (0: autotuned)
   int64_t fused_waves = kDefaultFusedWaves;  // target wave count that sets the fused j split
   int64_t pf_waves = kDefaultPfWaves;        // target wave count that sets the force Q groups
   bool autotune = true;
@@ -812,8 +813,8 @@ int tune_fq(pyiid_ctx* c, int slot, double qbin) {
 }
 
 int tune_fused(pyiid_ctx* c, int slot, double qbin) {
-  if (!c->autotune) {
-    c->grad_m = kGradM;
+  if (c->grad_m_fixed || !c->autotune) {
+    c->grad_m = c->grad_m_fixed ? c->grad_m_fixed : kGradM;
     c->fused_waves = kDefaultFusedWaves;
     return PYIID_OK;
   }
@@ -1452,6 +1453,14 @@ int pyiid_set_fq_bins_per_wave(pyiid_ctx* c, int m) {
   if (m != 0 && m != 64 && m != 72 && m != 80 && m != 88 && m != 96 && m != 112)
     return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 64, 72, 80, 88, 96 or 112", m);
   c->fq_m_fixed = m;
+  return PYIID_OK;
+}
+
+int pyiid_set_grad_bins_per_wave(pyiid_ctx* c, int m) {
+  CHK(check_ctx(c));
+  if (m != 0 && m != 16 && m != 32)
+    return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 16 or 32", m);
+  c->grad_m_fixed = m;
   return PYIID_OK;
 }
 
