@@ -1,9 +1,9 @@
 """Samplers driven by the MI355X Calc1D.
 
-* :func:`leapfrog` mirrors ``pyiid/sim/__init__.py:10-38`` (velocity Verlet on
-  an atoms object with an attached calculator);
-* :func:`simulate_dynamics`, :func:`mh_accept`, :func:`hmc_move`,
-  :func:`run_hmc` mirror ``pyiid/sim/hmc.py:10-209``;
+* :func:`leapfrog` (velocity Verlet on an atoms object with an attached
+  calculator) and :func:`simulate_dynamics`, :func:`mh_accept`,
+  :func:`hmc_move`, :func:`run_hmc` keep the interfaces and behaviour of
+  ``pyiid/sim/__init__.py:10-38`` and ``pyiid/sim/hmc.py:33-209``;
 * :class:`DeviceHMC` runs the same Hamiltonian Monte Carlo for a batch of
   independent structures entirely on the device (``pyiid_hmc_setup`` /
   ``pyiid_hmc_run``): every force evaluation, leapfrog update and Metropolis
@@ -19,74 +19,117 @@ from .formfactors import MASSES, element_table
 __all__ = ["leapfrog", "simulate_dynamics", "mh_accept", "hmc_move", "run_hmc", "DeviceHMC"]
 
 
-def leapfrog(atoms, step, center=True):
-    """One velocity-Verlet step on a copy of ``atoms`` (sim/__init__.py:10-38):
-    p += step/2 F; x += step v; p += step/2 F(x'); optional centring."""
-    latoms = dc(atoms) if not hasattr(atoms, "copy") else atoms.copy()
-    latoms.set_momenta(latoms.get_momenta() + 0.5 * step * latoms.get_forces())
-    latoms.positions += step * latoms.get_velocities()
-    latoms.set_momenta(latoms.get_momenta() + 0.5 * step * latoms.get_forces())
-    if center:
-        latoms.center()
-    return latoms
-
-
-def _lf(atoms, step):
-    """The inner step of hmc.py:10-30 (kick with the full step, then drift by
-    the momenta)."""
+def _kick(atoms, h, velocities=False):
+    """Momenta (or, with ``velocities``, velocities) += h F."""
     f = atoms.get_forces()
-    atoms.set_momenta(atoms.get_momenta() + step * f)
-    atoms.set_positions(atoms.get_positions() + step * atoms.get_momenta())
+    if velocities:
+        atoms.set_velocities(atoms.get_velocities() + h * f)
+    else:
+        atoms.set_momenta(atoms.get_momenta() + h * f)
+
+
+def _drift(atoms, h, by_momenta=False):
+    """Positions += h v (or h p with ``by_momenta``)."""
+    rate = atoms.get_momenta() if by_momenta else atoms.get_velocities()
+    atoms.set_positions(atoms.get_positions() + h * rate)
+
+
+def leapfrog(atoms, step, center=True):
+    """One velocity-Verlet step of size ``step`` on a copy of ``atoms``: half
+    kick, drift by the velocities, half kick at the new positions, then
+    optionally re-centre (interface of pyiid/sim/__init__.py:10-38).  The
+    input is left unchanged."""
+    moved = atoms.copy() if hasattr(atoms, "copy") else dc(atoms)
+    _kick(moved, 0.5 * step)
+    _drift(moved, step)
+    _kick(moved, 0.5 * step)
+    if center:
+        moved.center()
+    return moved
 
 
 def simulate_dynamics(atoms, stepsize, n_steps):
-    """hmc.py:33-63."""
-    grad = atoms.get_forces()
-    atoms.set_velocities(atoms.get_velocities() + 0.5 * stepsize * grad)
-    atoms.set_positions(atoms.get_positions() + stepsize * atoms.get_velocities())
-    for _ in range(n_steps):
-        _lf(atoms, stepsize)
-    atoms.set_velocities(atoms.get_velocities() + 0.5 * stepsize * atoms.get_forces())
+    """Move ``atoms`` in place along the Hamiltonian flow for n_steps + 1
+    position updates (pyiid/sim/hmc.py:33-63).  pyIID's schedule is kept as
+    it is: the opening and closing half steps add F to the velocities (the
+    force taken as an acceleration) and the first drift moves by the
+    velocities, while the n_steps inner steps add F to the momenta and move
+    by the momenta.  With unit masses this is velocity Verlet."""
+    h = float(stepsize)
+    _kick(atoms, 0.5 * h, velocities=True)
+    _drift(atoms, h)
+    for _ in range(int(n_steps)):
+        _kick(atoms, h)
+        _drift(atoms, h, by_momenta=True)
+    _kick(atoms, 0.5 * h, velocities=True)
 
 
 def mh_accept(initial_energy, next_energy, temp=1, rs=np.random):
-    """Metropolis rule of hmc.py:66-92: accept when u < exp((H0 - H1) temp)."""
-    diff = initial_energy - next_energy
-    return rs.random_sample((1,)) < np.exp(diff * temp)
+    """Metropolis decision on total energies (pyiid/sim/hmc.py:66-92): True
+    with probability min(1, exp((H0 - H1) temp)); ``temp`` multiplies the
+    energy difference as in pyIID.  Returns a 1-element boolean array."""
+    with np.errstate(over="ignore"):
+        return rs.random_sample((1,)) < np.exp((initial_energy - next_energy) * temp)
 
 
 def hmc_move(atoms, stepsize, n_steps, temp, rs=np.random):
-    """hmc.py:95-130 (velocities N(0,1)/3/N, as the reference draws them)."""
+    """One HMC proposal (pyiid/sim/hmc.py:95-130): velocities drawn as
+    N(0, 1) / (3 N) per component (pyIID's choice), dynamics on a copy, then
+    the Metropolis decision.  Returns (accepted, state to continue from)."""
     atoms.set_velocities(rs.normal(0, 1, (len(atoms), 3)) / 3 / len(atoms))
-    prop = atoms.copy()
-    simulate_dynamics(prop, stepsize, n_steps)
-    accept = mh_accept(atoms.get_total_energy(), prop.get_total_energy(), temp, rs)
-    if accept:
-        return True, prop
+    trial = atoms.copy()
+    simulate_dynamics(trial, stepsize, n_steps)
+    if mh_accept(atoms.get_total_energy(), trial.get_total_energy(), temp, rs):
+        return True, trial
     return False, atoms
+
+
+class _AcceptanceStep(object):
+    """pyIID's step-size rule for run_hmc: a running acceptance rate, relaxed
+    towards the mean of every decision so far with time constant
+    ``slowness``, grows the step by ``inc`` above ``target`` and shrinks it by
+    ``dec`` otherwise, within [lo, hi]."""
+
+    def __init__(self, step, rate, slowness, target, inc, dec, lo, hi):
+        self.step, self.rate, self.slowness, self.target = step, rate, slowness, target
+        self.inc, self.dec, self.lo, self.hi = inc, dec, lo, hi
+        self.n_seen = 0
+        self.n_accepted = 0
+
+    def record(self, accepted):
+        self.n_seen += 1
+        self.n_accepted += int(bool(accepted))
+        mean = self.n_accepted / float(self.n_seen)
+        self.rate = self.slowness * self.rate + (1.0 - self.slowness) * mean
+        self.step *= self.inc if self.rate > self.target else self.dec
+        self.step = min(max(self.step, self.lo), self.hi)
+        return self.step
 
 
 def run_hmc(atoms, iterations, stepsize, n_steps, avg_acceptance_slowness, avg_acceptance_rate,
             target_acceptance_rate, stepsize_inc, stepsize_dec, stepsize_min, stepsize_max,
             temp=1, wtraj=None, rs=np.random):
-    """Adaptive-step HMC of hmc.py:133-209; returns (traj, accept_list, accept_num)."""
-    traj = [atoms]
-    accept_list, accept_num = [], []
-    i = 0
-    while i < iterations:
-        accept, atoms = hmc_move(atoms, stepsize, n_steps, temp, rs)
-        if accept is True:
-            traj += [atoms]
-            accept_num.append(i)
-            if wtraj is not None:
-                wtraj.write(atoms)
-        accept_list.append(accept)
-        avg_acceptance_rate = avg_acceptance_slowness * avg_acceptance_rate + \
-            (1.0 - avg_acceptance_slowness) * np.average(np.array(accept_list))
-        stepsize *= stepsize_inc if avg_acceptance_rate > target_acceptance_rate else stepsize_dec
-        stepsize = min(max(stepsize, stepsize_min), stepsize_max)
-        i += 1
-    return traj, accept_list, accept_num
+    """HMC with an adapted step size (interface of pyiid/sim/hmc.py:133-209).
+    Returns (trajectory: the start then every accepted state, the decision of
+    every iteration, the iterations that accepted)."""
+    adapt = _AcceptanceStep(stepsize, avg_acceptance_rate, avg_acceptance_slowness,
+                            target_acceptance_rate, stepsize_inc, stepsize_dec, stepsize_min,
+                            stepsize_max)
+    traj, decisions, accepted_at = [atoms], [], []
+    state = atoms
+    try:
+        for it in range(int(iterations)):
+            ok, state = hmc_move(state, adapt.step, n_steps, temp, rs)
+            decisions.append(ok)
+            if ok:
+                traj.append(state)
+                accepted_at.append(it)
+                if wtraj is not None:
+                    wtraj.write(state)
+            adapt.record(ok)
+    except KeyboardInterrupt:
+        pass
+    return traj, decisions, accepted_at
 
 
 class DeviceHMC(object):
