@@ -756,6 +756,8 @@ int tune_contract(pyiid_ctx* c, int slot, double qbin) {
     c->pf_waves = hit->second[0];
     return PYIID_OK;
   }
+  // a shape first met under graph capture runs the default shape
+  c->pf_waves = kDefaultPfWaves;
   if (c->capturing) return PYIID_OK;
   double best = 1e300;
   int64_t best_w = c->pf_waves;
@@ -827,6 +829,9 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
     c->fused_waves = hit->second[1];
     return PYIID_OK;
   }
+  // a shape first met under graph capture runs the default shape
+  c->grad_m = kGradM;
+  c->fused_waves = kDefaultFusedWaves;
   if (c->capturing) return PYIID_OK;
   double best = 1e300;
   int best_m = c->grad_m;
@@ -1442,8 +1447,11 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
   return PYIID_OK;
 }
 
+// The launch-shape setters below drop captured proposals when the setting
+// changes: a graph records the shapes it was captured with.
 int pyiid_set_autotune(pyiid_ctx* c, int enable) {
   CHK(check_ctx(c));
+  if (c->autotune != (enable != 0)) destroy_graph(c->hmc);
   c->autotune = enable != 0;
   return PYIID_OK;
 }
@@ -1452,6 +1460,7 @@ int pyiid_set_fq_bins_per_wave(pyiid_ctx* c, int m) {
   CHK(check_ctx(c));
   if (m != 0 && m != 64 && m != 72 && m != 80 && m != 88 && m != 96 && m != 112)
     return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 64, 72, 80, 88, 96 or 112", m);
+  if (c->fq_m_fixed != m) destroy_graph(c->hmc);
   c->fq_m_fixed = m;
   return PYIID_OK;
 }
@@ -1460,6 +1469,7 @@ int pyiid_set_grad_bins_per_wave(pyiid_ctx* c, int m) {
   CHK(check_ctx(c));
   if (m != 0 && m != 16 && m != 32)
     return fail(PYIID_E_INVALID, "bins per wave %d is not 0, 16 or 32", m);
+  if (c->grad_m_fixed != m) destroy_graph(c->hmc);
   c->grad_m_fixed = m;
   return PYIID_OK;
 }

@@ -149,17 +149,25 @@ constexpr float kChebEpsMin = 0.039866844f;
 // reused as the transpose buffer of the end-of-kernel lane reduction.
 constexpr int kRedRow = 65;  // 64 lanes + 1 pad (bank spread)
 constexpr int kRedM = 16;    // Q bins reduced per pass
-constexpr int kStoreRow = kGradM + 4;  // 16-byte aligned rows for the store transpose
 constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-// Fused kernel: per round of kGeoJ j atoms the block's waves (same atoms,
-// different Q chunks) share the per-pair setup through LDS (FP32,
-// [field][k][lane]): 1/r, (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi and
-// 4 lo, and the anchor sin, cos(Q0 r) of every wave's Q chunk.
-constexpr int kGeoJ = 16;
-constexpr int kGeoFields = 7 + 2 * kFusedWaves;
-constexpr int kGeoBytes = kGeoJ * 64 * kGeoFields * 4;
-constexpr int kFusedSmemBytes = cmax3(kFusedWaves * kRedM * kRedRow * 4,
-                                      kFusedWaves * 64 * kStoreRow * 4, kJTile * 32 + kGeoBytes);
+// Fused kernel shapes: a block covers kFusedBins Q bins of one 64-atom tile
+// as W = kFusedBins / M waves of M consecutive bins: M = 32 in 8 waves
+// (2 per SIMD, 237 VGPRs) or M = 16 in 16 waves (4 per SIMD within 128
+// VGPRs).  Per round of GJ j atoms the block's waves (same atoms, different Q
+// chunks) share the per-pair setup through LDS (FP32, [field][k][lane]): 1/r,
+// (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, and the anchor
+// sin, cos(Q0 r) of every wave's Q chunk.
+constexpr int kFusedBins = 256;
+template <int M>
+struct FusedShape {
+  static constexpr int W = kFusedBins / M;        // waves per block
+  static constexpr int GJ = W == 8 ? 16 : 8;      // j atoms per setup round
+  static constexpr int H = W / 8;                 // setup tasks per pair (8 anchors each)
+  static constexpr int kFields = 7 + 2 * W;
+  static constexpr int kRow = M + 4;              // store-transpose row (16-byte aligned)
+  static constexpr int kSmem = cmax3(W * kRedM * kRedRow * 4, W * 64 * kRow * 4,
+                                     kJTile * 32 + GJ * 64 * kFields * 4);
+};
 // F-only kernel: rounds of kFqGeoJ j atoms share the per-pair setup (FP32,
 // [field][k][lane]): 1/r, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, eps of d,
 // (-1)^k (dtheta = k pi + d), and the anchor sin, cos(Q0 r) of each of the
@@ -207,6 +215,7 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
                                            int lane, float* base, size_t row_stride, int n_rows,
                                            int n_valid) {
   static_assert(64 % M == 0 && M <= kGradM, "tile width");
+  constexpr int kStoreRow = M + 4;
   float* buf = reinterpret_cast<float*>(smem) + wave * 64 * kStoreRow;
   __syncthreads();
 #pragma unroll
@@ -225,7 +234,7 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 
 // ---------------------------------------------------------------------------
 // Fused F + gradient kernel (ordered pairs).
-//   grid: x = work item (i tile x j range), y = Q group (kFusedWaves waves x M bins)
+//   grid: x = work item (i tile x j range), y = Q group (kFusedBins bins: W waves x M bins)
 //   gpart[split][i][c][q]  float  sum_j t_ij(q) (q_i - q_j)_c / r^2 (ff-weighted)
 //   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
 // Weighted: the item's j indices are positions in jperm (the atoms in
@@ -233,20 +242,22 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 // the single-element one and f_i(Q) f_j(Q) scales the lane's sums at the end.
 // ---------------------------------------------------------------------------
 template <int M, bool WEIGHTED>
-__global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
+__global__ __launch_bounds__(64 * FusedShape<M>::W) void fused_grad_kernel(
     const AtomRec* __restrict__ pos, const int32_t* __restrict__ jperm, int n_total,
     const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
     float* __restrict__ gpart, double* __restrict__ fpart) {
-  __shared__ __attribute__((aligned(32))) unsigned char smem[kFusedSmemBytes];
+  using S = FusedShape<M>;
+  constexpr int W = S::W, GJ = S::GJ, H = S::H;
+  __shared__ __attribute__((aligned(32))) unsigned char smem[S::kSmem];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
-  constexpr int kGS = kGeoJ * 64;  // field stride in geo_f
+  constexpr int kGS = GJ * 64;  // field stride in geo_f
 
   const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = it.i_base + lane;
-  const int qchunk = blockIdx.y * kFusedWaves + wave;
+  const int qchunk = blockIdx.y * W + wave;
   const int q0 = qchunk * M;
   const int j_begin = it.jbeg;
   const int j_end = it.jend;
@@ -267,38 +278,43 @@ __global__ __launch_bounds__(64 * kFusedWaves) void fused_grad_kernel(
   for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = 0.0f;
 
   // first bin of the block (its waves take consecutive chunks of M bins)
-  const double qbase = static_cast<double>(blockIdx.y * kFusedWaves * M);
+  const double qbase = static_cast<double>(blockIdx.y * kFusedBins);
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
     if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end)
       tile[threadIdx.x] = pos[WEIGHTED ? jperm[jt + threadIdx.x] : jt + threadIdx.x];
     __syncthreads();
     const int cnt = min(kJTile, j_end - jt);
-    for (int js = 0; js < cnt; js += kGeoJ) {
-      const int kk = min(kGeoJ, cnt - js);
-      // shared setup: wave w forms the pairs of j = js + w, js + w + 4, ...
-      for (int k = wave; k < kk; k += kFusedWaves) {
+    for (int js = 0; js < cnt; js += GJ) {
+      const int kk = min(GJ, cnt - js);
+      // shared setup: task t = (pair k, anchor part) of the round, wave w
+      // takes tasks w, w + W, ...; part p writes the anchors of chunks
+      // 8 p .. 8 p + 7 (part 0 also the pair's geometry fields)
+      for (int t = wave; t < kk * H; t += W) {
+        const int k = t / H, part = t - k * H;
         const PairGeom g = pair_geom(pi, tile[js + k], lane_ok);
         const double dth = dq * g.r;
-        float s1, k2hi, k2lo;
-        two_cos(dth, k2hi, k2lo, s1);
         const int o = k * 64 + lane;
-        geo_f[o] = g.inv_r;
-        geo_f[o + kGS] = g.dpx;
-        geo_f[o + 2 * kGS] = g.dpy;
-        geo_f[o + 3 * kGS] = g.dpz;
-        geo_f[o + 4 * kGS] = s1;
-        geo_f[o + 5 * kGS] = k2hi;
-        geo_f[o + 6 * kGS] = 4.0f * k2lo;
-        // the anchors sin, cos(Q0 r) of the block's kFusedWaves Q chunks: the
-        // first reduced in FP64, the others by FP32 rotations by M dtheta
+        if (part == 0) {
+          float s1, k2hi, k2lo;
+          two_cos(dth, k2hi, k2lo, s1);
+          geo_f[o] = g.inv_r;
+          geo_f[o + kGS] = g.dpx;
+          geo_f[o + 2 * kGS] = g.dpy;
+          geo_f[o + 3 * kGS] = g.dpz;
+          geo_f[o + 4 * kGS] = s1;
+          geo_f[o + 5 * kGS] = k2hi;
+          geo_f[o + 6 * kGS] = 4.0f * k2lo;
+        }
+        // the anchors sin, cos(Q0 r) of the part's 8 Q chunks: the first
+        // reduced in FP64, the others by FP32 rotations by M dtheta
         float as, ac, rs, rc;
-        sincos_red(qbase * dth, as, ac);
+        sincos_red((qbase + static_cast<double>(8 * part * M)) * dth, as, ac);
         sincos_red(static_cast<double>(M) * dth, rs, rc);
 #pragma unroll
-        for (int w = 0; w < kFusedWaves; ++w) {
-          geo_f[o + (7 + 2 * w) * kGS] = as;
-          geo_f[o + (8 + 2 * w) * kGS] = ac;
+        for (int w = 0; w < 8; ++w) {
+          geo_f[o + (7 + 2 * (8 * part + w)) * kGS] = as;
+          geo_f[o + (8 + 2 * (8 * part + w)) * kGS] = ac;
           const float as2 = fmaf(as, rc, ac * rs);
           ac = fmaf(ac, rc, -as * rs);
           as = as2;
@@ -974,8 +990,8 @@ template <int M>
 hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, const int32_t* jperm, int n_total,
                           const float* ftab, int nq, double dq, const Item* items, int n_items,
                           bool weighted, float* gpart, double* fpart) {
-  dim3 grid(n_items, (nq + kFusedWaves * M - 1) / (kFusedWaves * M));
-  dim3 block(64 * kFusedWaves);
+  dim3 grid(n_items, (nq + kFusedBins - 1) / kFusedBins);
+  dim3 block(64 * FusedShape<M>::W);
   if (weighted)
     hipLaunchKernelGGL((fused_grad_kernel<M, true>), grid, block, 0, st, pos, jperm, n_total, ftab,
                        nq, dq, items, gpart, fpart);

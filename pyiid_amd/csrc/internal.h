@@ -18,7 +18,6 @@ struct alignas(32) AtomRec {
 constexpr int kGradM = 32;
 constexpr int kFqM = 64;  // default Q bins per wave of the F-only kernel
 constexpr int kWaves = 4;      // waves per block in the contracted-force kernel
-constexpr int kFusedWaves = 8; // waves per block in the fused F + gradient kernel
 constexpr int kJTile = 256;    // atoms staged in LDS per j tile
 
 // Work item of the Debye kernels.  Structures ("chains") are stored back to

@@ -228,22 +228,35 @@ def test_fq_every_bins_per_wave(ctx, m):
         ctx.set_fq_bins_per_wave(0)
 
 
-@pytest.mark.parametrize("m", [16, 32])
-def test_fused_every_bins_per_wave(ctx, m):
-    """Each launch shape of the fused F + gradient kernel (m bins per wave,
-    fixed instead of autotuned) against the oracle: 203 atoms (rounds of 16 j
-    atoms end in an odd count, so the two-register-set loop runs its tail) at
-    330 bins, and a Pt/Ni alloy of 170 atoms at 700 bins."""
+def test_fused_every_bins_per_wave(ctx):
+    """Each launch shape of the fused F + gradient kernel (m bins per wave:
+    32 in blocks of 8 waves, 16 in blocks of 16 waves), fixed instead of
+    autotuned, against the oracle: 203 atoms (the 32-bin shape's rounds of 16
+    j atoms end in an odd count, so the two-register-set loop runs its tail;
+    the 16-bin shape's rounds of 8 split the tile at another place) at 330
+    bins, and a Pt/Ni alloy of 170 atoms at 700 bins.  The two shapes anchor
+    the recurrences at different bins, so their float32 results must differ
+    in the last bits: equal outputs would mean the setting did not reach the
+    launch."""
+    cases = ((203, 330, 0.0756865, False), (170, 700, 0.05, True))
+    out = {}
     try:
-        ctx.set_grad_bins_per_wave(m)
-        for n, nq, qbin, alloy in ((203, 330, 0.0756865, False), (170, 700, 0.05, True)):
-            pos = structures.jitter(structures.fcc_nearest(n, a=3.9), 0.05, seed=n + m).astype(np.float32)
-            numbers = structures.random_alloy(n, 78, 28, seed=n) if alloy else np.full(n, 79)
-            per_atom = load(ctx, pos, numbers, qbin, nq)
-            fq_out = np.empty(nq, np.float32)
-            got = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_out, f64=True)
-            assert rel(got, oracle.grad_fq(pos, per_atom, qbin)) < TOL
-            assert rel(fq_out, oracle.fq(pos, per_atom, qbin)) < TOL
+        for m in (16, 32):
+            ctx.set_grad_bins_per_wave(m)
+            for n, nq, qbin, alloy in cases:
+                pos = structures.jitter(structures.fcc_nearest(n, a=3.9), 0.05,
+                                        seed=n).astype(np.float32)
+                numbers = structures.random_alloy(n, 78, 28, seed=n) if alloy else np.full(n, 79)
+                per_atom = load(ctx, pos, numbers, qbin, nq)
+                fq_out = np.empty(nq, np.float32)
+                got = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_out)
+                eg = rel(got, oracle.grad_fq(pos, per_atom, qbin))
+                ef = rel(fq_out, oracle.fq(pos, per_atom, qbin))
+                print("m=%d n=%d: grad %.2e F %.2e" % (m, n, eg, ef))
+                assert eg < TOL and ef < TOL
+                out[(m, n)] = got
+        for n in (203, 170):
+            assert not np.array_equal(out[(16, n)], out[(32, n)])
         with pytest.raises(ValueError, match="bins per wave"):
             ctx.set_grad_bins_per_wave(24)
     finally:
