@@ -247,6 +247,35 @@ int pyiid_sur_hmc_run(pyiid_ctx *ctx, int n_proposals, int n_leapfrog, double st
                       double min_r2, double *energies, int32_t *accepted, int32_t *exact_mode,
                       double *times_ms, int32_t *violations, int flags);
 
+/* ---- F(Q) without device allocation ------------------------------------ */
+#define PYIID_NOALLOC_MAX_ATOMS 65536
+#define PYIID_NOALLOC_MAX_ELEMENTS 32
+#define PYIID_NOALLOC_MAX_BINS 4096
+#define PYIID_NOALLOC_SCRATCH_MIB 37
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory, e.g. code under HIP graph capture, tight serving loops, or
+ * harnesses that forbid hipMalloc.
+ *
+ * Computes F(Q) of one structure as pyiid_fq does.  There is no context and
+ * no device allocation.
+ *  - Arguments.  pos (float32 n x 3), elem (int32 n scatter-table rows;
+ *    NULL means all 0), table (float32 n_elem x n_q) and fq (float32 n_q)
+ *    are device pointers on `device`.
+ *  - Stream.  The work is ordered on `stream` (a hipStream_t; NULL is the
+ *    null stream), and the call returns without waiting for it.
+ *  - Scratch.  Intermediates live in a static device array of at most
+ *    PYIID_NOALLOC_SCRATCH_MIB MiB on each GPU, held until the process ends.
+ *    Every call on a GPU shares that array, so calls that overlap on one GPU
+ *    (on other streams or threads) race.
+ *  - Limits.  Beyond the PYIID_NOALLOC_MAX_* limits the call returns
+ *    PYIID_E_INVALID.  An element index outside [0, n_elem) makes every bin
+ *    of fq NaN.
+ *  - First call.  The first call on a device looks the array up; make it
+ *    before capturing a graph.
+ *  - Restores the caller's current device. */
+int pyiid_fq_noalloc(int device, void *stream, const float *pos, const int32_t *elem, int64_t n,
+                     const float *table, int32_t n_elem, int32_t n_q, double qbin, float *fq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ _SIGNATURES = [
     ("pyiid_sur_train", [_P, _D, _D, _P, _I]),
     ("pyiid_sur_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _I, _D, _D, _D, _P, _P, _P,
                            _P, _P, _I]),
+    ("pyiid_fq_noalloc", [_I, _P, _P, _P, _I64, _P, _I32, _I32, _D, _P]),
 ]
 EXPORTED = [name for name, _ in _SIGNATURES] + ["pyiid_last_error"]
 

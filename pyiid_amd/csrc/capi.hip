@@ -1351,6 +1351,14 @@ struct RunClock {
 
 }  // namespace
 
+int pyiid::set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
 extern "C" {
 
 int pyiid_abi_version(void) { return PYIID_ABI_VERSION; }

@@ -512,6 +512,8 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
   constexpr int kGS = kFqGeoJ * 64;
 
   const Item it = items[blockIdx.x];
+  // an empty item (padding of a list sized on the host) writes nothing
+  if (it.jbeg >= it.jend) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = it.i_base + lane;

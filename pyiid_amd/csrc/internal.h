@@ -159,4 +159,7 @@ hipError_t launch_gram(hipStream_t st, const float* phi, int64_t n3, const doubl
 hipError_t launch_disp_check(hipStream_t st, const AtomRec* pos, const double* pos_save,
                              const int* atom_chain, int64_t n, double lim, int* viol);
 
+// Records the calling thread's message for pyiid_last_error and returns code.
+int set_error(int code, const char* fmt, ...);
+
 }  // namespace pyiid
