@@ -152,19 +152,17 @@ class Calc1D(Calculator):
         scat._ensure(atoms)
         gobs = np.asarray(self.target_data, dtype=np.float64)
         if fname == "get_pdf":
-            st, slot, nq = _proc._setup(atoms, "PDF")
-            with st.lock:
+            if len(gobs) != len(scat.get_r()):
+                return None
+            with _proc.loaded(atoms, "PDF") as (st, slot, nq):
                 st.load_transform(scat.exp["rstep"], scat.pdf_qbin, scat.get_r(), scat.exp["qmin"],
                                   nq)
-                if len(gobs) != len(scat.get_r()):
-                    return None
                 return st.ctx.pdf_energy_forces(slot, scat.pdf_qbin, gobs, self._pot,
                                                 self.rw_to_eV, len(atoms), want_forces)
-        st, slot, nq = _proc._setup(atoms, "fq")
-        first = int(np.floor(scat.exp["qmin"] / scat.exp["qbin"]))
-        if len(gobs) != nq - first:
-            return None
-        with st.lock:
+        with _proc.loaded(atoms, "fq") as (st, slot, nq):
+            first = int(np.floor(scat.exp["qmin"] / scat.exp["qbin"]))
+            if len(gobs) != nq - first:
+                return None
             return st.ctx.fq_energy_forces(slot, scat.exp["qbin"], first, gobs, self._pot,
                                            self.rw_to_eV, len(atoms), want_forces)
 

@@ -26,6 +26,25 @@ from .formfactors import element_table, form_factor
 all_changes = ["positions", "numbers", "cell", "pbc", "charges", "magmoms", "exp"]
 
 
+def project_to_detector(q, iq, qbin, pixel_array):
+    """Detector image of I(Q): each pixel holds the Q it sees and receives
+    I(Q_k) of the bin whose open window (Q_k - qbin/2, Q_k + qbin/2) contains
+    it.  Where rounding makes two neighbouring windows overlap, the later bin
+    wins.  Pixels outside every window are 0.  These are the semantics of
+    ``__init__.py:448-475``, found per pixel by binary search over the
+    window edges rather than by a pass over the bins."""
+    q = np.asarray(q, dtype=np.float64)
+    pixels = np.asarray(pixel_array)
+    flat = pixels.ravel()
+    half = qbin / 2.
+    last = np.searchsorted(q - half, flat, side="left") - 1   # last k with Q_k - qbin/2 < p
+    first = np.searchsorted(q + half, flat, side="right")     # first k with Q_k + qbin/2 > p
+    hit = (last >= first) & (last >= 0)
+    img = np.zeros(flat.shape)
+    img[hit] = np.asarray(iq)[last[hit]]
+    return img.reshape(pixels.shape)
+
+
 class ElasticScatter(object):
     """Elastic powder scattering of a set of atoms: F(Q), S(Q), I(Q), G(r) and
     the gradients of F(Q) and G(r)."""
@@ -156,26 +175,36 @@ class ElasticScatter(object):
         first = int(np.floor(self.exp["qmin"] / self.exp["qbin"]))
         fq = fq[first:]
         if iq_std is not None:
-            fq_std = iq_std * np.abs(self.get_scatter_vector()) / np.abs(
-                np.average(atoms.get_array("F(Q) scatter"), axis=0) ** 2)[first:]
-            if fq_std[0] == 0.0:
-                fq_std[0] += 1e-9
-            fq += self.rs.normal(0, fq_std)
+            # the I(Q) width carried to F = Q (I / <f>^2 - 1): sigma_F = sigma_I Q / <f>^2
+            mean_f = atoms.get_array("F(Q) scatter").mean(axis=0)
+            width = iq_std * np.abs(self.get_scatter_vector()) / (mean_f * mean_f)[first:]
+            fq += self._noise(width)
         return fq
+
+    def _noise(self, width):
+        """One Gaussian draw per bin with the given widths, from the
+        experiment's seeded generator.  A zero width in the first bin (Q = 0)
+        becomes 1e-9, as in the reference (``__init__.py:337-338,378-379``)."""
+        width = np.array(width, dtype=np.float64)
+        if width[0] == 0.0:
+            width[0] = 1e-9
+        return self.rs.normal(0.0, width)
 
     def get_pdf(self, atoms, iq_std=None, noise_distribution=np.random.normal):
         """Atomic pair distribution function G(r) (``__init__.py:343-391``)."""
         self._ensure(atoms)
         fq = self.fq(atoms, self.pdf_qbin, "PDF")
         if iq_std is not None:
-            a = np.abs(self.get_scatter_vector(pdf=True))
-            b = np.abs(np.average(atoms.get_array("PDF scatter") ** 2, axis=0))
-            if hasattr(iq_std, "shape") and iq_std.shape != a.shape:
-                iq_std = np.interp(np.arange(a.shape[0]), np.arange(iq_std.shape[0]), iq_std)
-            fq_noise = iq_std * a / b
-            if fq_noise[0] == 0.0:
-                fq_noise[0] += 1e-9
-            fq += self.rs.normal(0, fq_noise)
+            q = np.abs(self.get_scatter_vector(pdf=True))
+            # here the reference divides by <f^2>, not <f>^2 (__init__.py:371-372)
+            f2 = atoms.get_array("PDF scatter")
+            mean_f2 = (f2 * f2).mean(axis=0)
+            if hasattr(iq_std, "shape") and iq_std.shape != q.shape:
+                # a width array on another grid is resampled linearly by bin
+                # index (the reference uses griddata, which gives NaN past the
+                # last point; np.interp holds the end value)
+                iq_std = np.interp(np.arange(len(q)), np.arange(len(iq_std)), iq_std)
+            fq += self._noise(iq_std * q / mean_f2)
         return _proc.pdf_at_qmin(fq, self.exp["rstep"], self.pdf_qbin, self.get_r(),
                                  self.exp["qmin"])
 
@@ -195,16 +224,8 @@ class ElasticScatter(object):
 
     def get_2d_scatter(self, atoms, pixel_array):
         """I(Q) projected onto a detector map of Q values (``__init__.py:448-475``)."""
-        iq = self.get_iq(atoms)
-        s = self.get_scatter_vector()
-        qb = self.exp["qbin"]
-        final_shape = pixel_array.shape
-        fp = pixel_array.ravel()
-        img = np.zeros(fp.shape)
-        for sub_s, i in zip(s, iq):
-            c = np.intersect1d(np.where(sub_s - qb / 2. < fp)[0], np.where(sub_s + qb / 2. > fp)[0])
-            img[c] = i
-        return img.reshape(final_shape)
+        return project_to_detector(self.get_scatter_vector(), self.get_iq(atoms),
+                                   self.exp["qbin"], pixel_array)
 
     def get_grad_fq(self, atoms):
         """Gradient of F(Q), reference convention -1/2 dF/dq (``__init__.py:477-496``)."""

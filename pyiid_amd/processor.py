@@ -14,6 +14,7 @@ All arithmetic runs in the gfx950 kernels of ``libpyiid_hip.so``; there is no
 CPU fallback.  ``sum_type == 'fq'`` selects the atoms' 'F(Q) scatter' table,
 anything else 'PDF scatter' (flat_multi_cpu_wrap.py:13-16).
 """
+import contextlib
 import math
 import threading
 
@@ -93,27 +94,31 @@ def _slot(sum_type):
     return _lib.SLOT_FQ if sum_type == "fq" else _lib.SLOT_PDF
 
 
-def _setup(atoms, sum_type, device=0):
+@contextlib.contextmanager
+def loaded(atoms, sum_type, device=0):
+    """Uploads the structure and the scatter table of ``atoms`` to the shared
+    context of ``device`` and yields (state, slot, n_q).  The context's lock
+    is held from the upload to the end of the ``with`` block, so another
+    thread cannot replace the structure before the caller's compute runs."""
     name = "F(Q) scatter" if sum_type == "fq" else "PDF scatter"
     table, index = compact_scatter(atoms, name)
-    st = state(device)
-    st.load_structure(atoms.get_positions(), index)
     slot = _slot(sum_type)
-    st.load_table(slot, table)
-    return st, slot, table.shape[1]
+    st = state(device)
+    with st.lock:
+        st.load_structure(atoms.get_positions(), index)
+        st.load_table(slot, table)
+        yield st, slot, table.shape[1]
 
 
 def wrap_fq(atoms, qbin=.1, sum_type="fq"):
     """Reduced structure factor F(Q) (float32, floor(qmax/qbin) bins)."""
-    st, slot, nq = _setup(atoms, sum_type)
-    with st.lock:
+    with loaded(atoms, sum_type) as (st, slot, nq):
         return st.ctx.fq(slot, qbin, nq)
 
 
 def wrap_fq_grad(atoms, qbin=.1, sum_type="fq"):
     """-1/2 dF/dq as an (N, 3, Q) float64 array (the reference convention)."""
-    st, slot, nq = _setup(atoms, sum_type)
-    with st.lock:
+    with loaded(atoms, sum_type) as (st, slot, nq):
         return st.ctx.grad_fq(slot, qbin, len(atoms), nq, f64=True)
 
 

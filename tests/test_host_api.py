@@ -84,3 +84,88 @@ def test_compat_formfactor_quirk(monkeypatch):
     for i, z in enumerate(e_list):
         got = atoms.get_array("F(Q) scatter")[np.array([28, 28, 79]) == z][0]
         np.testing.assert_allclose(got, rows[i].astype(np.float32))
+
+
+def _project_by_bins(q, iq, qbin, pixels):
+    """Detector projection bin by bin: every pixel strictly inside a bin's
+    window (Q_k - qbin/2, Q_k + qbin/2) takes that bin's I, later bins
+    overwriting earlier ones (pyiid/experiments/elasticscatter/__init__.py:
+    448-475)."""
+    flat = pixels.ravel()
+    img = np.zeros(flat.shape)
+    for qk, ik in zip(q, iq):
+        img[(qk - qbin / 2. < flat) & (qk + qbin / 2. > flat)] = ik
+    return img.reshape(pixels.shape)
+
+
+@pytest.mark.parametrize("qmin,qbin", [(0.0, 0.1), (0.7, 0.0756865), (1.3, 0.12)])
+def test_detector_projection_equals_bin_loop(qmin, qbin):
+    rs = np.random.RandomState(3)
+    q = np.arange(qmin, np.floor(25.0 / qbin) * qbin, qbin)
+    iq = rs.random_sample(len(q)) + 0.5
+    centres = rs.choice(q, 50)
+    edges = np.concatenate([q - qbin / 2., q + qbin / 2.])[rs.randint(0, 2 * len(q), 60)]
+    pix = np.concatenate([rs.uniform(-1.0, 27.0, 4096 - 110 - 4), centres, edges,
+                          [np.nan, -np.inf, np.inf, q[-1] + qbin / 2.]]).reshape(64, 64)
+    got = es.project_to_detector(q, iq, qbin, pix)
+    want = _project_by_bins(q, iq, qbin, pix)
+    np.testing.assert_array_equal(got, want)
+    # pixels just inside a window take that bin; outside every window, 0
+    k = len(q) // 2
+    inside = np.array([[q[k] - 0.49 * qbin, q[k] + 0.49 * qbin, q[0] - 0.51 * qbin]])
+    np.testing.assert_array_equal(es.project_to_detector(q, iq, qbin, inside),
+                                  [[iq[k], iq[k], 0.0]])
+
+
+def test_shared_context_serves_each_thread_its_own_structure(monkeypatch):
+    """Two threads share the per-device context.  Each must get F(Q) and the
+    gradient of the structure it passed.  The stand-in context answers from
+    the positions last uploaded and sleeps between upload and compute, so
+    without the lock held across both, the threads' structures would
+    interleave."""
+    import threading
+    import time
+
+    class Recording:
+        def __init__(self, device=0):
+            self.pos = None
+
+        def set_positions(self, pos):
+            self.pos = np.array(pos, np.float32)
+            time.sleep(0.001)
+
+        def set_elements(self, elem):
+            pass
+
+        def set_scatter(self, slot, table):
+            pass
+
+        def fq(self, slot, qbin, nq):
+            time.sleep(0.001)
+            return np.full(nq, self.pos.sum(), np.float32)
+
+        def grad_fq(self, slot, qbin, n, nq, f64=False):
+            time.sleep(0.001)
+            return np.full((n, 3, nq), self.pos.sum())
+
+    monkeypatch.setattr(processor._lib, "Context", Recording)
+    monkeypatch.setattr(processor, "_ctx", {})
+    structs = []
+    for k in range(2):
+        a = Atoms("Au%d" % (5 + k), np.random.RandomState(k).random_sample((5 + k, 3)) * 5)
+        a.set_array("F(Q) scatter", np.ones((len(a), 30), np.float32))
+        structs.append(a)
+    wrong = []
+
+    def work(a):
+        want = np.float32(np.asarray(a.get_positions(), np.float32).sum())
+        for _ in range(40):
+            if processor.wrap_fq(a)[0] != want or processor.wrap_fq_grad(a)[0, 0, 0] != want:
+                wrong.append(len(a))
+
+    threads = [threading.Thread(target=work, args=(a,)) for a in structs]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert wrong == []
