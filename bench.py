@@ -55,30 +55,213 @@ def read_traffic(n_pairs_q):
         return None
 
 
+def host_cpu():
+    """The host cores this process may use and the CPU model: the affinity
+    mask, capped by the cgroup's CPU quota when one is set (a job limited to
+    16 CPUs of a 128-core host gains nothing from more threads)."""
+    import math
+    import subprocess
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, period = fh.read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    cores = affinity if quota is None else max(1, min(affinity, int(math.floor(quota))))
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "cgroup_quota_cpus": quota, "threads_used": cores}
+
+
 def cpu_baseline(pos, table, index, target_s=15.0):
-    """The oracle's C restatement of the reference 'flat' CPU algorithm on the
-    host cores, on a strided subset of the Q bins (cost is linear in bins)."""
+    """The oracle's C restatement of the reference 'flat' CPU algorithm
+    (float32 pair terms, threaded over the flat pair list like the
+    reference's process pool) on every core this process may use, built with
+    -march=native on this host.  The C2 sample is a strided subset of the Q
+    bins (the cost is linear in bins).  C1 is timed in full, and so is exact
+    HMC at C1, each beside the device's time for the same work."""
+    import tempfile
     from oracle import native
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = host["threads_used"]
+    build_dir = tempfile.mkdtemp(prefix="pyiid_cpu_")
+    lib = native.build_native(build_dir)
     per_atom = table[index]
     n = len(pos)
     pairs = n * (n - 1) // 2
     probe = np.array([NQ // 2], np.int32)
     t0 = time.perf_counter()
-    native.flat_f32(pos, per_atom, QBIN, q_index=probe, nthreads=threads)
+    native.flat_f32(pos, per_atom, QBIN, q_index=probe, nthreads=threads, lib=lib)
     dt = time.perf_counter() - t0
     nsel = int(max(2, min(NQ, target_s / max(dt, 1e-3))))
     qi = np.unique(np.linspace(0, NQ - 1, nsel).round().astype(np.int32))
     t0 = time.perf_counter()
-    native.flat_f32(pos, per_atom, QBIN, q_index=qi, nthreads=threads)
+    native.flat_f32(pos, per_atom, QBIN, q_index=qi, nthreads=threads, lib=lib)
     dt = time.perf_counter() - t0
-    return {
+    out = {
         "value": pairs * len(qi) / dt,
         "unit": "pair·Q evals/s",
         "cores": threads,
         "kind": "port",
         "sample": "same structure, F+grad over all %d pairs at %d of the %d Q bins "
                   "(evenly strided), %.1f s" % (pairs, len(qi), NQ, dt),
+        "host": host,
+        "build": "gcc -O3 -march=native" if lib else "gcc -O3 -march=x86-64-v2 (native build failed)",
+    }
+    out["c1"] = c1_compare(lib, threads)
+    return out
+
+
+C1_EXP = {"qmin": 0.0, "qmax": 25.0, "qbin": 0.1, "rmin": 0.0, "rmax": 40.0, "rstep": 0.01}
+
+
+def _c1_cpu_energy_forces(lib, e, pq, per_pdf, gobs, conv, transform="gemm", workers=1):
+    """conv Rw and -dE/dq of the 55-atom particle on the host.
+    * F + grad F: the reference's float32 flat algorithm (C restatement, one
+      thread: 1,485 pairs).
+    * G(r) and grad G(r): ``transform`` "gemm" applies the dense matrix T of
+      the F -> G map (the oracle's transform_matrix, equal to the FFT path) as
+      one BLAS product over the 165 gradient rows; "fft" runs the reference's
+      FFT path row by row, batched on ``workers`` threads.
+    * Rw and its gradient."""
+    from oracle import native
+    from oracle import scatter as S
+    r = S.r_grid(e)
+    nq = per_pdf.shape[1]
+    tmat = S.transform_matrix(e["rstep"], pq, r, e["qmin"], nq) if transform == "gemm" else None
+
+    def run(x):
+        fp, gfp = native.flat_f32(x, per_pdf, pq, nthreads=1, lib=lib)
+        gfp = gfp.astype(np.float64)
+        gfp[:, :, :int(np.ceil(e["qmin"] / pq))] = 0.0
+        if tmat is not None:
+            g = tmat @ fp.astype(np.float64)
+            gg = (gfp.reshape(-1, nq) @ tmat.T).reshape(len(x), 3, len(r))
+        else:
+            g = S.get_pdf_at_qmin(fp.astype(np.float64), e["rstep"], pq, r, e["qmin"])
+            gg = S.grad_pdf_rows(gfp, e["rstep"], pq, r, e["qmin"], workers=workers)
+        rw = S.get_rw(gobs, g)[0]
+        # the reference's forces are half of -dE/dq (its -1/2 dF/dq convention)
+        return conv * rw, 2.0 * conv * S.wrap_grad_rw(gg, g, gobs)
+    return run
+
+
+def c1_compare(lib, threads, n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100.0):
+    """The 55-atom Au cuboctahedron (a = 4.08 A, sigma 0.05 A jitter, seed 0;
+    Q 0-25 step 0.1, r 0-40 step 0.01) in full on the host and on the device:
+    one evaluation of F(Q), grad F, G(r), grad G and Rw against the ideal
+    particle's G(r); and exact HMC (10 velocity-Verlet steps of 2e-3 per
+    proposal, ASE masses, energy 100 Rw, beta 50), seconds per proposal."""
+    from oracle import native
+    from oracle import scatter as S
+    from pyiid_amd import Atoms, structures
+    from pyiid_amd.calc import wrap_rw
+    from pyiid_amd.elasticscatter import ElasticScatter
+    from pyiid_amd.formfactors import MASSES, element_table
+    from pyiid_amd.sim import DeviceHMC
+    ideal = structures.cuboctahedron(4.08)
+    pos = structures.jitter(ideal, 0.05, 0)
+    n = len(pos)
+    e, pq = S.make_experiment(dict(C1_EXP))
+    nq = S.qmax_bin(e["qmax"], e["qbin"])
+    npq = S.qmax_bin(e["qmax"], pq)
+    zeros = np.zeros(n, int)
+    per_q = element_table(np.full(n, 79), e["qbin"], nq)[0][zeros]
+    per_p = element_table(np.full(n, 79), pq, npq)[0][zeros]
+    r = S.r_grid(e)
+    f_ideal, _ = native.flat_f32(ideal, per_p, pq, grad=False, lib=lib)
+    gobs = S.get_pdf_at_qmin(f_ideal.astype(np.float64), e["rstep"], pq, r, e["qmin"])
+    energy_forces = _c1_cpu_energy_forces(lib, e, pq, per_p, gobs, conv)
+    fft_energy_forces = _c1_cpu_energy_forces(lib, e, pq, per_p, gobs, conv, "fft", threads)
+
+    def cpu_eval():
+        native.flat_f32(pos, per_q, e["qbin"], nthreads=1, lib=lib)     # F(Q), grad F
+        return energy_forces(pos)                                       # G, grad G, Rw
+
+    def cpu_eval_fft():
+        native.flat_f32(pos, per_q, e["qbin"], nthreads=1, lib=lib)
+        return fft_energy_forces(pos)
+
+    def timed(fn, min_s=2.0):
+        fn()
+        k, t0 = 0, time.perf_counter()
+        while True:
+            fn()
+            k += 1
+            dt = time.perf_counter() - t0
+            if dt >= min_s:
+                return dt / k, k
+
+    cpu_s, cpu_k = timed(cpu_eval)
+    cpu_fft_s, _ = timed(cpu_eval_fft)
+    scat = ElasticScatter(dict(C1_EXP), seed=0)
+    atoms = Atoms("Au%d" % n, pos)
+    g_ideal = scat.get_pdf(Atoms("Au%d" % n, ideal))
+
+    def gpu_eval():
+        scat.get_fq(atoms)
+        scat.get_grad_fq(atoms)
+        g = scat.get_pdf(atoms)
+        scat.get_grad_pdf(atoms)
+        return wrap_rw(g, g_ideal)
+
+    gpu_s, gpu_k = timed(gpu_eval)
+
+    # exact HMC: momenta ~ N(0, m / beta), velocity Verlet with the force
+    # carried between steps, Metropolis on exp(beta (H0 - H1))
+    m = np.array([MASSES[79]] * n)
+    rs = np.random.RandomState(0)
+    x = np.asarray(pos, np.float32).astype(np.float64)
+    ex, fx = energy_forces(x)
+    acc = 0
+    t0 = time.perf_counter()
+    for _ in range(n_prop):
+        p = rs.normal(0.0, 1.0, (n, 3)) * np.sqrt(m / beta)[:, None]
+        h0 = ex + np.sum(p * p / (2 * m[:, None]))
+        y, f, q = x.copy(), fx.copy(), p.copy()
+        for _ in range(n_leap):
+            q += 0.5 * eps * f
+            y += eps * q / m[:, None]
+            ey, f = energy_forces(y)
+            q += 0.5 * eps * f
+        h1 = ey + np.sum(q * q / (2 * m[:, None]))
+        if rs.random_sample() < np.exp(min(0.0, beta * (h0 - h1))):
+            x, ex, fx, acc = y, ey, f, acc + 1
+    cpu_hmc_s = (time.perf_counter() - t0) / n_prop
+    hmc = DeviceHMC(scat, [(pos, np.full(n, 79))], g_ideal[None, :], conv=conv)
+    hmc.run(2, n_leapfrog=n_leap, step=eps, beta=beta, seed=1)      # capture
+    _, a_dev, t = hmc.run(4 * n_prop, n_leapfrog=n_leap, step=eps, beta=beta, seed=1,
+                          times=True)
+    hmc.close()
+    gpu_hmc_s = float(t[-1]) / 1e3 / (4 * n_prop)
+    return {
+        "workload": "55-atom Au cuboctahedron (a=4.08 A), sigma=0.05 A seeded jitter; "
+                    "Q 0-25 step 0.1, r 0-40 step 0.01; Rw against the ideal particle",
+        "eval": "F(Q), grad F, G(r), grad G, Rw",
+        "cpu_s_per_eval": cpu_s, "cpu_evals": cpu_k,
+        "cpu_transform": "dense T (R x Q) applied by one BLAS product over the gradient rows",
+        "cpu_s_per_eval_fft_rows": cpu_fft_s,
+        "cpu_threads": "1 for the pair sums (1,485 pairs); BLAS / %d FFT workers for the "
+                       "transform" % threads,
+        "gpu_s_per_eval": gpu_s, "gpu_evals": gpu_k,
+        "gpu_eval_path": "ElasticScatter getters (host arrays, PCIe round trips included)",
+        "hmc": {"leapfrog_steps": n_leap, "step": eps, "beta": beta, "conv": conv,
+                "masses": "ASE (Au 196.97 amu)",
+                "cpu_s_per_proposal": cpu_hmc_s, "cpu_proposals": n_prop,
+                "cpu_acceptance": acc / n_prop,
+                "gpu_s_per_proposal": gpu_hmc_s, "gpu_proposals": 4 * n_prop,
+                "gpu_acceptance": float(a_dev.mean())},
     }
 
 

@@ -7,15 +7,15 @@ import numpy as np
 
 _PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build",
                      "libdebye_oracle.so")
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_PATH):
-            raise RuntimeError("oracle C library not built (make -C oracle)")
-        lib = ctypes.CDLL(_PATH)
+def load(path=_PATH):
+    """The C restatement at ``path`` (default: the portable build)."""
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError("oracle C library not built (make -C oracle): %s" % path)
+        lib = ctypes.CDLL(path)
         lib.oracle_flat_f32.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                         ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
@@ -26,13 +26,29 @@ def load():
                                                                  ctypes.c_int, ctypes.c_void_p,
                                                                  ctypes.c_void_p]
         lib.oracle_sums_f64.restype = ctypes.c_int
-        _lib = lib
-    return _lib
+        _libs[path] = lib
+    return _libs[path]
 
 
-def flat_f32(q, scatter, qbin, q_index=None, nthreads=1, grad=True):
+def build_native(outdir):
+    """Builds the C restatement with -march=native for this machine into
+    ``outdir`` (the Makefile's ``native`` target) and returns its path, or
+    None when the build fails."""
+    import subprocess
+    path = os.path.join(outdir, "libdebye_oracle_native.so")
+    here = os.path.dirname(os.path.abspath(__file__))
+    try:
+        subprocess.run(["make", "-s", "-C", here, "native", "NATIVE=" + path], check=True,
+                       capture_output=True, timeout=120)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    return path if os.path.exists(path) else None
+
+
+def flat_f32(q, scatter, qbin, q_index=None, nthreads=1, grad=True, lib=None):
     """(F at q_index, grad at q_index or None) with the reference's float32
-    per-pair arithmetic (cpu_flat.py + flat_multi_cpu_wrap.py)."""
+    per-pair arithmetic (cpu_flat.py + flat_multi_cpu_wrap.py); ``lib``: a
+    path to another build of the same source."""
     q = np.ascontiguousarray(q, dtype=np.float32)
     f = np.ascontiguousarray(scatter, dtype=np.float32)
     n, nq = f.shape
@@ -40,7 +56,7 @@ def flat_f32(q, scatter, qbin, q_index=None, nthreads=1, grad=True):
         np.ascontiguousarray(q_index, dtype=np.int32)
     fq = np.empty(len(qi), np.float32)
     g = np.empty((n, 3, len(qi)), np.float32) if grad else None
-    rc = load().oracle_flat_f32(q.ctypes.data, f.ctypes.data, n, nq, float(np.float32(qbin)),
+    rc = load(lib or _PATH).oracle_flat_f32(q.ctypes.data, f.ctypes.data, n, nq, float(np.float32(qbin)),
                                 qi.ctypes.data, len(qi), int(nthreads), fq.ctypes.data,
                                 None if g is None else g.ctypes.data)
     if rc != 0:

@@ -352,6 +352,45 @@ def grad_pdf(grad_fq_arr, rstep, qstep, rgrid, qmin):
     return out
 
 
+def get_pdf_at_qmin_rows(f_rows, rstep, qstep, rgrid, qmin, workers=1, chunk=128):
+    """:func:`get_pdf_at_qmin` of every row of ``f_rows`` (rows x nq): the
+    same zeroing, padding, odd-extended inverse FFT and interpolation, run as
+    batched FFTs over ``chunk`` rows at a time on ``workers`` threads
+    (scipy.fft).  The reference spreads the rows of its gradient transform
+    over a process pool instead (``master_kernel.py:272-290``)."""
+    import scipy.fft
+    f_rows = np.array(f_rows, dtype=np.float64, ndmin=2)
+    rows = f_rows.shape[0]
+    f_rows[:, :int(math.ceil(qmin / qstep))] = 0.0
+    length = max(f_rows.shape[1], int(math.ceil(math.pi / rstep / qstep)))
+    npad1 = int(round(qmin / qstep)) + length
+    npad2 = (1 << int(math.ceil(math.log(npad1, 2)))) * 2
+    npad4 = 4 * npad2
+    drpad = math.pi / (npad2 * qstep)
+    x = np.asarray(rgrid, dtype=np.float64) / drpad / 2
+    lo = x.astype(np.int64)
+    w = x - lo
+    m = np.arange(1, f_rows.shape[1])
+    out = np.empty((rows, len(x)))
+    for r0 in range(0, rows, chunk):
+        blk = f_rows[r0:r0 + chunk]
+        ext = np.zeros((len(blk), npad4))
+        ext[:, 0:2 * blk.shape[1]:2] = blk
+        ext[:, npad4 - 2 * m] = -blk[:, 1:]
+        spec = scipy.fft.ifft(ext, axis=1, workers=workers)
+        gpad = (spec[:, 0:2 * npad2:2] * npad2 * qstep).imag * (2.0 / math.pi)
+        out[r0:r0 + chunk] = 2.0 * ((1.0 - w) * gpad[:, lo] + w * gpad[:, lo + 1])
+    return out
+
+
+def grad_pdf_rows(grad_fq_arr, rstep, qstep, rgrid, qmin, workers=1):
+    """:func:`grad_pdf` through :func:`get_pdf_at_qmin_rows`."""
+    g = np.asarray(grad_fq_arr, dtype=np.float64)
+    n = g.shape[0]
+    flat = get_pdf_at_qmin_rows(g.reshape(n * 3, -1), rstep, qstep, rgrid, qmin, workers)
+    return flat.reshape(n, 3, len(rgrid))
+
+
 # ---------------------------------------------------------------------------
 # Rw / chi^2 and gradients: master_kernel.py:206-266,293-396,
 # calc/__init__.py:10-105

@@ -138,3 +138,20 @@ def test_leapfrog_invariants():
     xb, pb = S.leapfrog(xa, -pa, m, harmonic, 0.1)
     np.testing.assert_allclose(xb, x, atol=1e-12)
     np.testing.assert_allclose(-pb, p, atol=1e-12)
+
+
+@pytest.mark.parametrize("qmin,rmin,rstep", [(0.0, 0.0, 0.01), (0.7, 1.2, 0.013)])
+def test_batched_rows_equal_row_by_row(qmin, rmin, rstep):
+    """The batched transform (rows through one FFT call, two threads, chunks
+    that do not divide the row count) against the row-by-row restatement."""
+    rs = np.random.RandomState(4)
+    e, pq = S.make_experiment({"qmin": qmin, "rmin": rmin, "rstep": rstep, "rmax": 30.0})
+    nq = S.qmax_bin(e["qmax"], pq)
+    r = S.r_grid(e)
+    g = rs.normal(size=(7, 3, nq))
+    want = S.grad_pdf(g.copy(), e["rstep"], pq, r, e["qmin"])
+    got = S.grad_pdf_rows(g, e["rstep"], pq, r, e["qmin"], workers=2)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.max(np.abs(want)))
+    rows = S.get_pdf_at_qmin_rows(g.reshape(21, nq), e["rstep"], pq, r, e["qmin"], chunk=5)
+    np.testing.assert_allclose(rows.reshape(7, 3, -1), want, rtol=0,
+                               atol=1e-12 * np.max(np.abs(want)))
