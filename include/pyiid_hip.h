@@ -87,6 +87,18 @@ int pyiid_set_grad_bins_per_wave(pyiid_ctx *ctx, int m);
  * results are the same bit for bit from run to run and context to context;
  * with it on they agree to FP32 rounding.  No counterpart in the reference. */
 int pyiid_set_autotune(pyiid_ctx *ctx, int enable);
+/* Summation of the Debye kernels' FP32 per-bin sums.
+ *  - PYIID_PRECISION_DEFAULT: plain FP32 sums over runs of at most 2,048
+ *    j atoms, then FP64 across runs and tiles.
+ *  - PYIID_PRECISION_COMPENSATED: compensated FP32 sums.  The terms of each
+ *    256-atom j tile gather in a second register, which an error-free TwoSum
+ *    folds into the running sum.  The F + gradient kernel then runs 16 bins
+ *    per wave in 8-wave blocks.
+ * The reference sums each pair chunk in FP64 (atomics/cpu_atomics.py:78);
+ * this is the FP32 device counterpart of that choice. */
+#define PYIID_PRECISION_DEFAULT 0
+#define PYIID_PRECISION_COMPENSATED 1
+int pyiid_set_precision(pyiid_ctx *ctx, int mode);
 
 /* ---- structure and tables ---------------------------------------------- */
 /* Atom positions, float32 N x 3 (the reference casts positions to float32:

@@ -18,6 +18,8 @@ SLOT_FQ = 0
 SLOT_PDF = 1
 POT_RW = 0
 POT_CHI_SQ = 1
+PRECISION_DEFAULT = 0
+PRECISION_COMPENSATED = 1
 
 KERNEL_FUSED = 0
 KERNEL_FQ = 1
@@ -51,6 +53,7 @@ _SIGNATURES = [
     ("pyiid_set_fq_bins_per_wave", [_P, _I]),
     ("pyiid_set_grad_bins_per_wave", [_P, _I]),
     ("pyiid_set_autotune", [_P, _I]),
+    ("pyiid_set_precision", [_P, _I]),
     ("pyiid_set_positions", [_P, _P, _I64, _I]),
     ("pyiid_set_elements", [_P, _P, _I64, _I]),
     ("pyiid_set_scatter", [_P, _I, _P, _I32, _I32, _I]),
@@ -182,6 +185,12 @@ class Context:
         """Autotune launch shapes per problem shape (default), or use fixed
         shapes (bit-reproducible results)."""
         check(self.lib.pyiid_set_autotune(self.handle, 1 if on else 0))
+
+    def set_precision(self, compensated):
+        """Compensated FP32 sums in the Debye kernels (True), or plain FP32
+        runs of at most 2,048 j atoms with FP64 across runs (False, default)."""
+        check(self.lib.pyiid_set_precision(self.handle, PRECISION_COMPENSATED if compensated
+                                           else PRECISION_DEFAULT))
 
     def set_fq_bins_per_wave(self, m):
         """Fix the F-only kernel's Q bins per wave (64, 72, 80, 88, 96, 112), or 0 to autotune."""

@@ -150,17 +150,30 @@ constexpr float kChebEpsMin = 0.039866844f;
 constexpr int kRedRow = 65;  // 64 lanes + 1 pad (bank spread)
 constexpr int kRedM = 16;    // Q bins reduced per pass
 constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
-// Fused kernel shapes: a block covers kFusedBins Q bins of one 64-atom tile
-// as W = kFusedBins / M waves of M consecutive bins: M = 32 in 8 waves
-// (2 per SIMD, 237 VGPRs) or M = 16 in 16 waves (4 per SIMD within 128
-// VGPRs).  Per round of GJ j atoms the block's waves (same atoms, different Q
-// chunks) share the per-pair setup through LDS (FP32, [field][k][lane]): 1/r,
-// (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, and the anchor
-// sin, cos(Q0 r) of every wave's Q chunk.
-constexpr int kFusedBins = 256;
-template <int M>
+// Fused kernel shapes: a block covers M W Q bins of one 64-atom tile as W
+// waves of M consecutive bins: M = 32 in 8 waves (2 per SIMD, 237 VGPRs),
+// M = 16 in 16 waves (4 per SIMD within 128 VGPRs), or, with compensated
+// sums, M = 16 in 8 waves.  Per round of GJ j atoms the block's waves (same
+// atoms, different Q chunks) share the per-pair setup through LDS (FP32,
+// [field][k][lane]): 1/r, (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi
+// and 4 lo, and the anchor sin, cos(Q0 r) of every wave's Q chunk.
+// hi + lo -> hi, with the rounding error of that addition left in lo
+// (TwoSum: exact for any magnitudes, no branches).
+__device__ __forceinline__ void two_sum_fold(float& hi, float& lo) {
+  const float s = hi + lo;
+  const float bb = s - hi;
+  lo = (hi - (s - bb)) + (lo - bb);
+  hi = s;
+}
+
+// Launch shapes of the fused kernel: M bins per wave, WAVES waves per block
+// (M WAVES bins per block).  Plain sums: 32 x 8 and 16 x 16 (256 bins);
+// compensated sums: 16 x 8 (128 bins), which leaves the registers for a
+// second set of accumulators.
+template <int M, int WAVES>
 struct FusedShape {
-  static constexpr int W = kFusedBins / M;        // waves per block
+  static constexpr int W = WAVES;                 // waves per block
+  static constexpr int kBins = M * W;             // bins per block
   static constexpr int GJ = W == 8 ? 16 : 8;      // j atoms per setup round
   static constexpr int H = W / 8;                 // setup tasks per pair (8 anchors each)
   static constexpr int kFields = 7 + 2 * W;
@@ -234,19 +247,25 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 
 // ---------------------------------------------------------------------------
 // Fused F + gradient kernel (ordered pairs).
-//   grid: x = work item (i tile x j range), y = Q group (kFusedBins bins: W waves x M bins)
+//   grid: x = work item (i tile x j range), y = Q group (W waves x M bins)
 //   gpart[split][i][c][q]  float  sum_j t_ij(q) (q_i - q_j)_c / r^2 (ff-weighted)
 //   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
 // Weighted: the item's j indices are positions in jperm (the atoms in
 // element order) and its j range is of one element, item.pad; the loop is
 // the single-element one and f_i(Q) f_j(Q) scales the lane's sums at the end.
 // ---------------------------------------------------------------------------
-template <int M, bool WEIGHTED>
-__global__ __launch_bounds__(64 * FusedShape<M>::W) void fused_grad_kernel(
+// COMP: compensated sums.  Every accumulator has a second FP32 register
+// that takes the terms of the current j tile; at the end of the tile an
+// error-free TwoSum moves it into the first and leaves the rounding error of
+// that addition in the second (Knuth, TAOCP vol. 2, 4.2.2).  The FP32
+// rounding then acts on sums of at most kJTile terms plus the carried error,
+// not on a running sum over the whole j range.
+template <int M, int WAVES, bool WEIGHTED, bool COMP>
+__global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
     const AtomRec* __restrict__ pos, const int32_t* __restrict__ jperm, int n_total,
     const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
     float* __restrict__ gpart, double* __restrict__ fpart) {
-  using S = FusedShape<M>;
+  using S = FusedShape<M, WAVES>;
   constexpr int W = S::W, GJ = S::GJ, H = S::H;
   __shared__ __attribute__((aligned(32))) unsigned char smem[S::kSmem];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
@@ -274,11 +293,16 @@ __global__ __launch_bounds__(64 * FusedShape<M>::W) void fused_grad_kernel(
   }
 
   float aF[M], ax[M], ay[M], az[M];
+  float bF[M], bx[M], by[M], bz[M];  // the tile's terms (COMP only)
 #pragma unroll
-  for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = 0.0f;
+  for (int m = 0; m < M; ++m) aF[m] = ax[m] = ay[m] = az[m] = bF[m] = bx[m] = by[m] = bz[m] = 0.0f;
+  float (&sF)[M] = COMP ? bF : aF;
+  float (&sx)[M] = COMP ? bx : ax;
+  float (&sy)[M] = COMP ? by : ay;
+  float (&sz)[M] = COMP ? bz : az;
 
   // first bin of the block (its waves take consecutive chunks of M bins)
-  const double qbase = static_cast<double>(blockIdx.y * kFusedBins);
+  const double qbase = static_cast<double>(blockIdx.y * S::kBins);
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
     if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end)
@@ -351,10 +375,10 @@ __global__ __launch_bounds__(64 * FusedShape<M>::W) void fused_grad_kernel(
 #pragma unroll
           for (int m = 0; m < M; ++m) {
             const float t = fmaf(qv[m], cs, -w);
-            aF[m] += w;
-            ax[m] = fmaf(t, dpx, ax[m]);
-            ay[m] = fmaf(t, dpy, ay[m]);
-            az[m] = fmaf(t, dpz, az[m]);
+            sF[m] += w;
+            sx[m] = fmaf(t, dpx, sx[m]);
+            sy[m] = fmaf(t, dpy, sy[m]);
+            sz[m] = fmaf(t, dpz, sz[m]);
             float wn = fmaf(k2, w, -wp);
             float cn = fmaf(k2, cs, -cp);
             if ((m & 3) == 3) {
@@ -380,6 +404,24 @@ __global__ __launch_bounds__(64 * FusedShape<M>::W) void fused_grad_kernel(
         if (k < kk) bins(ga);
       }
       __syncthreads();
+    }
+    if constexpr (COMP) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) {
+        two_sum_fold(aF[m], bF[m]);
+        two_sum_fold(ax[m], bx[m]);
+        two_sum_fold(ay[m], by[m]);
+        two_sum_fold(az[m], bz[m]);
+      }
+    }
+  }
+  if constexpr (COMP) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      aF[m] += bF[m];
+      ax[m] += bx[m];
+      ay[m] += by[m];
+      az[m] += bz[m];
     }
   }
 
@@ -499,9 +541,12 @@ __device__ __forceinline__ void fq_reinsch(float (&acc)[M], const float* geo_f, 
   }
 }
 
-// M bins per wave (a multiple of 8, at most 128); at least 3 waves per SIMD.
-template <int M, bool WEIGHTED>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(3))) void
+// M bins per wave (a multiple of 8, at most 128); at least 3 waves per SIMD
+// (2 with COMP, whose second accumulator set takes M more registers).
+// COMP: compensated sums as in fused_grad_kernel, the terms of each j tile
+// gathered in a second accumulator and folded in by TwoSum.
+template <int M, bool WEIGHTED, bool COMP>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(COMP ? 2 : 3))) void
 fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
           const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
           double* __restrict__ fpart) {
@@ -526,9 +571,10 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
   const bool lane_ok = i < it.i_end;
   const AtomRec pi = pos[perm[lane_ok ? i : it.i_base]];
 
-  float acc[M];
+  float acc[M], lo[M];
 #pragma unroll
-  for (int m = 0; m < M; ++m) acc[m] = 0.0f;
+  for (int m = 0; m < M; ++m) acc[m] = lo[m] = 0.0f;
+  float (&sum)[M] = COMP ? lo : acc;
 
   // first bin of the block (its waves take consecutive chunks of M bins)
   const double qbase = static_cast<double>(blockIdx.y * kWaves * M);
@@ -599,18 +645,26 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
         uint32_t left = three;
         for (; __builtin_popcount(left) >= 2; left &= left - 1, left &= left - 1) {
           const int ka = __builtin_ctz(left), kb = __builtin_ctz(left & (left - 1));
-          fq_three_term<M, 2>(acc, geo_f, kGS, {ka * 64 + lane, kb * 64 + lane}, chunk);
+          fq_three_term<M, 2>(sum, geo_f, kGS, {ka * 64 + lane, kb * 64 + lane}, chunk);
         }
-        if (left != 0) fq_three_term<M, 1>(acc, geo_f, kGS, {__builtin_ctz(left) * 64 + lane}, chunk);
+        if (left != 0) fq_three_term<M, 1>(sum, geo_f, kGS, {__builtin_ctz(left) * 64 + lane}, chunk);
         left = ~three & ((1u << kk) - 1);
         for (; __builtin_popcount(left) >= 2; left &= left - 1, left &= left - 1) {
           const int ka = __builtin_ctz(left), kb = __builtin_ctz(left & (left - 1));
-          fq_reinsch<M, 2>(acc, geo_f, kGS, {ka * 64 + lane, kb * 64 + lane}, chunk);
+          fq_reinsch<M, 2>(sum, geo_f, kGS, {ka * 64 + lane, kb * 64 + lane}, chunk);
         }
-        if (left != 0) fq_reinsch<M, 1>(acc, geo_f, kGS, {__builtin_ctz(left) * 64 + lane}, chunk);
+        if (left != 0) fq_reinsch<M, 1>(sum, geo_f, kGS, {__builtin_ctz(left) * 64 + lane}, chunk);
       }
       __syncthreads();
     }
+    if constexpr (COMP) {
+#pragma unroll
+      for (int m = 0; m < M; ++m) two_sum_fold(acc[m], lo[m]);
+    }
+  }
+  if constexpr (COMP) {
+#pragma unroll
+    for (int m = 0; m < M; ++m) acc[m] += lo[m];
   }
 
   // lane l keeps bin q0 + l, and bin q0 + 64 + l when M > 64; weighted: the
@@ -988,31 +1042,38 @@ int grid_for(int64_t total, int block) {
 
 }  // namespace
 
-template <int M>
+template <int M, int WAVES, bool COMP>
 hipError_t launch_fused_m(hipStream_t st, const AtomRec* pos, const int32_t* jperm, int n_total,
                           const float* ftab, int nq, double dq, const Item* items, int n_items,
                           bool weighted, float* gpart, double* fpart) {
-  dim3 grid(n_items, (nq + kFusedBins - 1) / kFusedBins);
-  dim3 block(64 * FusedShape<M>::W);
+  constexpr int kBins = FusedShape<M, WAVES>::kBins;
+  dim3 grid(n_items, (nq + kBins - 1) / kBins);
+  dim3 block(64 * WAVES);
   if (weighted)
-    hipLaunchKernelGGL((fused_grad_kernel<M, true>), grid, block, 0, st, pos, jperm, n_total, ftab,
-                       nq, dq, items, gpart, fpart);
+    hipLaunchKernelGGL((fused_grad_kernel<M, WAVES, true, COMP>), grid, block, 0, st, pos, jperm,
+                       n_total, ftab, nq, dq, items, gpart, fpart);
   else
-    hipLaunchKernelGGL((fused_grad_kernel<M, false>), grid, block, 0, st, pos, jperm, n_total,
-                       ftab, nq, dq, items, gpart, fpart);
+    hipLaunchKernelGGL((fused_grad_kernel<M, WAVES, false, COMP>), grid, block, 0, st, pos, jperm,
+                       n_total, ftab, nq, dq, items, gpart, fpart);
   return hipGetLastError();
 }
 
-hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
-                             int n_total, const float* ftab, int nq, double dq, const Item* items,
-                             int n_items, bool weighted, float* gpart, double* fpart) {
+hipError_t launch_fused_grad(hipStream_t st, int m, bool compensated, const AtomRec* pos,
+                             const int32_t* jperm, int n_total, const float* ftab, int nq,
+                             double dq, const Item* items, int n_items, bool weighted,
+                             float* gpart, double* fpart) {
   if (n_items <= 0) return hipSuccess;
+  if (compensated) {
+    if (m != 16) return hipErrorInvalidValue;
+    return launch_fused_m<16, 8, true>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items,
+                                       weighted, gpart, fpart);
+  }
   if (m == 16)
-    return launch_fused_m<16>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items, weighted,
-                              gpart, fpart);
+    return launch_fused_m<16, 16, false>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items,
+                                         weighted, gpart, fpart);
   if (m == 32)
-    return launch_fused_m<32>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items, weighted,
-                              gpart, fpart);
+    return launch_fused_m<32, 8, false>(st, pos, jperm, n_total, ftab, nq, dq, items, n_items,
+                                        weighted, gpart, fpart);
   return hipErrorInvalidValue;
 }
 
@@ -1052,19 +1113,27 @@ hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double
   return hipGetLastError();
 }
 
-template <int M, bool WEIGHTED>
+template <int M, bool WEIGHTED, bool COMP = false>
 static void launch_fq_m(hipStream_t st, const AtomRec* pos, const int32_t* perm,
                         const float* ftab, int nq, double dq, const Item* items, int n_items,
                         double* fpart) {
   dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
-  hipLaunchKernelGGL((fq_kernel<M, WEIGHTED>), grid, dim3(64 * kWaves), 0, st, pos, perm, ftab,
-                     nq, dq, items, fpart);
+  hipLaunchKernelGGL((fq_kernel<M, WEIGHTED, COMP>), grid, dim3(64 * kWaves), 0, st, pos, perm,
+                     ftab, nq, dq, items, fpart);
 }
 
-hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* perm,
-                     const float* ftab, int nq, double dq, const Item* items, int n_items,
-                     bool weighted, double* fpart) {
+hipError_t launch_fq(hipStream_t st, int m, bool compensated, const AtomRec* pos,
+                     const int32_t* perm, const float* ftab, int nq, double dq, const Item* items,
+                     int n_items, bool weighted, double* fpart) {
   if (n_items <= 0) return hipSuccess;
+  if (compensated) {
+    if (m != 64) return hipErrorInvalidValue;
+    if (weighted)
+      launch_fq_m<64, true, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    else
+      launch_fq_m<64, false, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+    return hipGetLastError();
+  }
   if (weighted) {
     if (m == 64)
       launch_fq_m<64, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);

@@ -35,14 +35,16 @@ struct Range {
 };
 
 // ---- launch wrappers (debye.hip) ----
-// Ordered-pair F + gradient, m = 16 or 32 Q bins per wave:
+// Ordered-pair F + gradient, m = 16 or 32 Q bins per wave (compensated
+// sums: m = 16 only):
 // gpart[item.split][i][c][q] (float) for i in the item's tile and
 // fpart[item][q] (F partial, double).  Weighted: the j indices of an item are
 // positions in jperm and item.pad is the element of the item's j range;
 // unweighted: atom indices.
-hipError_t launch_fused_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* jperm,
-                             int n_total, const float* ftab, int nq, double dq, const Item* items,
-                             int n_items, bool weighted, float* gpart, double* fpart);
+hipError_t launch_fused_grad(hipStream_t st, int m, bool compensated, const AtomRec* pos,
+                             const int32_t* jperm, int n_total, const float* ftab, int nq,
+                             double dq, const Item* items, int n_items, bool weighted,
+                             float* gpart, double* fpart);
 // Contracted forces over unordered pairs: items are tile pairs (I, J), J <= I,
 // of at most 64 positions of one chain in perm (Item.tile = I's and Item.pad =
 // J's chain-global tile index; weighted: Item.split = e_I | e_J << 16, each
@@ -62,12 +64,12 @@ hipError_t launch_reduce_pair_force(hipStream_t st, const float* part, const int
 // uq[b][q] = u[b][q] * float32(q dq)
 hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double dq, float* uq);
 // Unordered-pair F only: fpart[item][q] (double), m Q bins per wave (64, 72,
-// 80, 88, 96 or 112).  Item indices are positions in perm; weighted:
-// item.split is the element of the item's i tile and item.pad that of its j
-// range.
-hipError_t launch_fq(hipStream_t st, int m, const AtomRec* pos, const int32_t* perm,
-                     const float* ftab, int nq, double dq, const Item* items, int n_items,
-                     bool weighted, double* fpart);
+// 80, 88, 96 or 112; compensated sums: 64 only).  Item indices are positions
+// in perm; weighted: item.split is the element of the item's i tile and
+// item.pad that of its j range.
+hipError_t launch_fq(hipStream_t st, int m, bool compensated, const AtomRec* pos,
+                     const int32_t* perm, const float* ftab, int nq, double dq, const Item* items,
+                     int n_items, bool weighted, double* fpart);
 // out[r][q] = scale[r][q] * sum_{p in ranges[r]} part[p][q], r < n_rows (FP64)
 hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range* ranges,
                                 int n_rows, int nq, const double* scale, double* out_f64,

@@ -69,3 +69,39 @@ def test_gradient_rows_beyond_one_grid_column():
                                   nthreads=min(16, os.cpu_count() or 1))
     g_dev = grad[atoms][:, :, q_idx]
     assert float(np.max(np.abs(g_dev - g_ref)) / np.max(np.abs(g_ref))) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["sphere", "alloy"])
+def test_compensated_sums_match_oracle(case):
+    """Compensated FP32 sums (pyiid_set_precision): F from both kernels and
+    the gradient against the FP64 oracle, single-element and weighted, on
+    bins and atoms across the grid."""
+    if case == "sphere":
+        pos = structures.jitter(structures.fcc_sphere(16.0), 0.05, seed=4).astype(np.float32)
+        numbers = np.full(len(pos), 79)
+        qbin, nq = 0.05, 500
+    else:
+        pos = structures.jitter(structures.fcc_nearest(3000, a=3.724), 0.05,
+                                seed=5).astype(np.float32)
+        numbers = structures.random_alloy(len(pos), 78, 28, seed=6)
+        qbin, nq = 0.02, 1500
+    n = len(pos)
+    table, index, _ = element_table(numbers, qbin, nq)
+    ctx = _lib.Context(0)
+    ctx.set_precision(True)
+    ctx.set_positions(pos)
+    ctx.set_elements(index)
+    ctx.set_scatter(_lib.SLOT_FQ, table)
+    fq = np.empty(nq, np.float32)
+    grad = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, f64=True, fq_out=fq)
+    fq_only = ctx.fq(_lib.SLOT_FQ, qbin, nq)
+    ctx.close()
+    q_idx = np.unique(np.linspace(1, nq - 1, 12).astype(int))
+    atoms = np.linspace(0, n - 1, 16).astype(int)
+    f_ref, g_ref = native.fp64_subset(pos, index, table, qbin, q_idx, atoms,
+                                      nthreads=min(16, os.cpu_count() or 1))
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))  # noqa: E731
+    errs = (rel(fq[q_idx], f_ref), rel(fq_only[q_idx], f_ref),
+            rel(grad[atoms][:, :, q_idx], g_ref))
+    print("compensated %s N=%d: F fused %.2e, F-only %.2e, grad %.2e" % ((case, n) + errs))
+    assert max(errs) < 1e-5
