@@ -1666,7 +1666,7 @@ int pyiid_grad_fq(pyiid_ctx* c, int slot, double qbin, void* grad, float* fq, in
     float* fdst = (flags & PYIID_DEVICE) ? fq : c->io_a.as<float>();
     HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(),
                                 c->fused_items.ranges.as<Range>() + c->fused_items.n_ranges, 1,
-                                nq, t.fso.as<double>(), nullptr, fdst));
+                                nq, t.fsu.as<double>(), nullptr, fdst));
     if (!(flags & PYIID_DEVICE)) CHK(download(c, fq, fdst, sizeof(float) * nq, flags));
   }
   return finish(c, flags);

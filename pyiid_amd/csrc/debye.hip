@@ -27,6 +27,8 @@
 //
 // Reference semantics (pyIID): cpu_flat.py:77-153, cpu_experimental.py:8-15,
 // flat_multi_cpu_wrap.py:22-102, gpu_flat.py:11-299.
+#include <type_traits>
+
 #include "internal.h"
 
 namespace pyiid {
@@ -176,7 +178,7 @@ struct FusedShape {
   static constexpr int kBins = M * W;             // bins per block
   static constexpr int GJ = W == 8 ? 16 : 8;      // j atoms per setup round
   static constexpr int H = W / 8;                 // setup tasks per pair (8 anchors each)
-  static constexpr int kFields = 7 + 2 * W;
+  static constexpr int kFields = 8 + 2 * W;
   static constexpr int kRow = M + 4;              // store-transpose row (16-byte aligned)
   static constexpr int kSmem = cmax3(W * kRedM * kRedRow * 4, W * 64 * kRow * 4,
                                      kJTile * 32 + GJ * 64 * kFields * 4);
@@ -249,7 +251,9 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 // Fused F + gradient kernel (ordered pairs).
 //   grid: x = work item (i tile x j range), y = Q group (W waves x M bins)
 //   gpart[split][i][c][q]  float  sum_j t_ij(q) (q_i - q_j)_c / r^2 (ff-weighted)
-//   fpart[item][q]         double sum_{i in tile} sum_j ff sin(Q r)/r
+//   fpart[item][q]         double sum_{i in tile} sum_{j < i} ff sin(Q r)/r (each
+//                          unordered pair once, from the item that holds its
+//                          j < i half)
 // Weighted: the item's j indices are positions in jperm (the atoms in
 // element order) and its j range is of one element, item.pad; the loop is
 // the single-element one and f_i(Q) f_j(Q) scales the lane's sums at the end.
@@ -260,19 +264,20 @@ __device__ __forceinline__ void store_tile(const float (&acc)[M], unsigned char*
 // that addition in the second (Knuth, TAOCP vol. 2, 4.2.2).  The FP32
 // rounding then acts on sums of at most kJTile terms plus the carried error,
 // not on a running sum over the whole j range.
-template <int M, int WAVES, bool WEIGHTED, bool COMP>
-__global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
+// WITH_F: the item holds j < i pairs, so it adds to F (each unordered pair
+// enters F once, from its j < i half, with the pair's weight 0 or 1); without,
+// the F sums are skipped and the item's F partial is written as 0.
+template <int M, int WAVES, bool WEIGHTED, bool COMP, bool WITH_F>
+__device__ __forceinline__ void fused_grad_body(
     const AtomRec* __restrict__ pos, const int32_t* __restrict__ jperm, int n_total,
-    const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
-    float* __restrict__ gpart, double* __restrict__ fpart) {
+    const float* __restrict__ ftab, int nq, double dq, const Item& it,
+    float* __restrict__ gpart, double* __restrict__ fpart, unsigned char* smem) {
   using S = FusedShape<M, WAVES>;
   constexpr int W = S::W, GJ = S::GJ, H = S::H;
-  __shared__ __attribute__((aligned(32))) unsigned char smem[S::kSmem];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
   float* geo_f = reinterpret_cast<float*>(smem + kJTile * 32);
   constexpr int kGS = GJ * 64;  // field stride in geo_f
 
-  const Item it = items[blockIdx.x];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = it.i_base + lane;
@@ -305,8 +310,12 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
   const double qbase = static_cast<double>(blockIdx.y * S::kBins);
   for (int jt = j_begin; jt < j_end; jt += kJTile) {
     __syncthreads();
-    if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end)
-      tile[threadIdx.x] = pos[WEIGHTED ? jperm[jt + threadIdx.x] : jt + threadIdx.x];
+    if ((int)threadIdx.x < kJTile && jt + (int)threadIdx.x < j_end) {
+      const int ja = WEIGHTED ? jperm[jt + threadIdx.x] : jt + threadIdx.x;
+      AtomRec rec = pos[ja];
+      rec.w = static_cast<double>(ja);  // the j atom's index (its element is item.pad)
+      tile[threadIdx.x] = rec;
+    }
     __syncthreads();
     const int cnt = min(kJTile, j_end - jt);
     for (int js = 0; js < cnt; js += GJ) {
@@ -316,7 +325,8 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
       // 8 p .. 8 p + 7 (part 0 also the pair's geometry fields)
       for (int t = wave; t < kk * H; t += W) {
         const int k = t / H, part = t - k * H;
-        const PairGeom g = pair_geom(pi, tile[js + k], lane_ok);
+        const AtomRec pj = tile[js + k];
+        const PairGeom g = pair_geom(pi, pj, lane_ok);
         const double dth = dq * g.r;
         const int o = k * 64 + lane;
         if (part == 0) {
@@ -329,6 +339,7 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
           geo_f[o + 4 * kGS] = s1;
           geo_f[o + 5 * kGS] = k2hi;
           geo_f[o + 6 * kGS] = 4.0f * k2lo;
+          geo_f[o + 7 * kGS] = static_cast<int>(pj.w) < i ? 1.0f : 0.0f;  // F weight
         }
         // the anchors sin, cos(Q0 r) of the part's 8 Q chunks: the first
         // reduced in FP64, the others by FP32 rotations by M dtheta
@@ -337,8 +348,8 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
         sincos_red(static_cast<double>(M) * dth, rs, rc);
 #pragma unroll
         for (int w = 0; w < 8; ++w) {
-          geo_f[o + (7 + 2 * (8 * part + w)) * kGS] = as;
-          geo_f[o + (8 + 2 * (8 * part + w)) * kGS] = ac;
+          geo_f[o + (8 + 2 * (8 * part + w)) * kGS] = as;
+          geo_f[o + (9 + 2 * (8 * part + w)) * kGS] = ac;
           const float as2 = fmaf(as, rc, ac * rs);
           ac = fmaf(ac, rc, -as * rs);
           as = as2;
@@ -350,18 +361,19 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
         // current atom's bins run.  With two waves per SIMD a wave stalled on
         // LDS leaves the SIMD single-issuing; the scheduling barriers keep the
         // reads ahead of the bins (about 3% faster than reading at first use).
-        float ga[9], gb[9];
-        auto load = [&](float (&g)[9], int k) {
+        float ga[10], gb[10];
+        auto load = [&](float (&g)[10], int k) {
           const int o = k * 64 + lane;
 #pragma unroll
-          for (int f = 0; f < 7; ++f) g[f] = geo_f[o + f * kGS];
-          g[7] = geo_f[o + (7 + 2 * wave) * kGS];
+          for (int f = 0; f < 8; ++f) g[f] = geo_f[o + f * kGS];
           g[8] = geo_f[o + (8 + 2 * wave) * kGS];
+          g[9] = geo_f[o + (9 + 2 * wave) * kGS];
         };
-        auto bins = [&](const float (&g)[9]) {
+        auto bins = [&](const float (&g)[10], auto with_f) {
+          constexpr bool kWithF = decltype(with_f)::value;
           const float inv_r = g[0], dpx = g[1], dpy = g[2], dpz = g[3], s1 = g[4], k2 = g[5],
-                      lo4 = g[6];
-          float sn = g[7], cs = g[8];
+                      lo4 = g[6], fw = g[7];
+          float sn = g[8], cs = g[9];
           // Along Q, w = sin(Q r) / r and cs = cos(Q r) follow the three-term
           // recurrence y' = 2 cos(dtheta) y - y_prev (one FMA each per bin)
           // with 2 cos(dtheta) = k2 + lo: the low part is added every fourth
@@ -375,7 +387,7 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
 #pragma unroll
           for (int m = 0; m < M; ++m) {
             const float t = fmaf(qv[m], cs, -w);
-            sF[m] += w;
+            if constexpr (kWithF) sF[m] = fmaf(w, fw, sF[m]);
             sx[m] = fmaf(t, dpx, sx[m]);
             sy[m] = fmaf(t, dpy, sy[m]);
             sz[m] = fmaf(t, dpz, sz[m]);
@@ -391,17 +403,18 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
             cs = cn;
           }
         };
+        const std::integral_constant<bool, WITH_F> with_f;
         load(ga, 0);
         int k = 0;
         for (; k + 1 < kk; k += 2) {
           load(gb, k + 1);
           __builtin_amdgcn_sched_barrier(0);
-          bins(ga);
+          bins(ga, with_f);
           load(ga, k + 2 < kk ? k + 2 : k + 1);
           __builtin_amdgcn_sched_barrier(0);
-          bins(gb);
+          bins(gb, with_f);
         }
-        if (k < kk) bins(ga);
+        if (k < kk) bins(ga, with_f);
       }
       __syncthreads();
     }
@@ -453,6 +466,28 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
   store_tile<M>(ax, smem, wave, lane, base, stride, n_rows, n_valid);
   store_tile<M>(ay, smem, wave, lane, base + nq, stride, n_rows, n_valid);
   store_tile<M>(az, smem, wave, lane, base + 2 * (size_t)nq, stride, n_rows, n_valid);
+}
+
+// An item whose j atoms are all at or above every i of its tile holds no
+// j < i pair and skips the F sums.  Within an item the j atom index grows
+// with the position (the unweighted j range is in atom order; a weighted one
+// lies in one element's run of the stable element order), so its first j
+// atom decides.  The two bodies share no live registers, so the branch costs
+// nothing in register allocation.
+template <int M, int WAVES, bool WEIGHTED, bool COMP>
+__global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
+    const AtomRec* __restrict__ pos, const int32_t* __restrict__ jperm, int n_total,
+    const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
+    float* __restrict__ gpart, double* __restrict__ fpart) {
+  __shared__ __attribute__((aligned(32))) unsigned char smem[FusedShape<M, WAVES>::kSmem];
+  const Item it = items[blockIdx.x];
+  const int j_first = it.jbeg < it.jend ? (WEIGHTED ? jperm[it.jbeg] : it.jbeg) : it.i_end;
+  if (j_first < it.i_end - 1)
+    fused_grad_body<M, WAVES, WEIGHTED, COMP, true>(pos, jperm, n_total, ftab, nq, dq, it, gpart,
+                                                    fpart, smem);
+  else
+    fused_grad_body<M, WAVES, WEIGHTED, COMP, false>(pos, jperm, n_total, ftab, nq, dq, it,
+                                                     gpart, fpart, smem);
 }
 
 // ---------------------------------------------------------------------------
