@@ -36,11 +36,11 @@ def workload():
     return pos, table, index
 
 
-def read_traffic(n_pairs_q):
-    """HBM bytes per launch of the fused kernel from the committed rocprof
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), or None when it has not been measured."""
-    path = os.path.join(ROOT, "profiles", "fused_grad_au6699_q1250_traffic.txt")
+def read_traffic(label="au6699_q1250"):
+    """HBM bytes per launch of the fused kernel on workload `label` from the
+    committed rocprof counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel calibration), or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "fused_grad_%s_traffic.txt" % label)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -265,6 +265,61 @@ def c1_compare(lib, threads, n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100
     }
 
 
+def headline_20k(steps=3, warmup=1):
+    """F(Q) + grad F(Q) on the two 20k-atom particles: a 20,113-atom Au FCC
+    sphere (r = 43.3 A, 1,250 bins of 0.02) and the 20,000-atom Pt/Ni random
+    alloy (50/50 seeded, nearest FCC sites at a = 3.724 A, 1,500 bins of
+    0.02), device-resident inputs and outputs.  Per workload: wall time per
+    step, the fused kernel's average launch (HIP events) and its algorithmic
+    fraction of FP32 peak."""
+    import torch
+    from pyiid_amd import _lib, structures
+    from pyiid_amd.formfactors import element_table
+    out = {}
+    cases = [
+        ("au20113_q1250", lambda: structures.fcc_sphere(43.3, LATTICE), lambda n: np.full(n, 79),
+         1250),
+        ("ptni20000_q1500", lambda: structures.fcc_nearest(20000, a=3.724),
+         lambda n: structures.random_alloy(n, 78, 28, seed=SEED), 1500),
+    ]
+    for label, sites, numbers, nq in cases:
+        pos = structures.jitter(sites(), SIGMA, SEED).astype(np.float32)
+        n = len(pos)
+        table, index, _ = element_table(numbers(n), QBIN, nq)
+        ctx = _lib.Context(0)
+        ctx.set_positions(pos)
+        ctx.set_elements(index)
+        ctx.set_scatter(_lib.SLOT_FQ, table)
+        g = torch.empty((n, 3, nq), dtype=torch.float32, device="cuda")
+        f = torch.empty(nq, dtype=torch.float32, device="cuda")
+
+        def step():
+            ctx.grad_fq(_lib.SLOT_FQ, QBIN, n, nq, out=g.data_ptr(), fq_out=f.data_ptr(),
+                        flags=_lib.PYIID_DEVICE)
+
+        for _ in range(warmup):
+            step()
+        ctx.synchronize()
+        ctx.enable_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        ms, k = ctx.kernel_time(_lib.KERNEL_FUSED)
+        ctx.enable_timing(False)
+        ctx.close()
+        pairs = n * (n - 1) // 2
+        avg_s = ms / max(1, k) / 1e3
+        achieved = FLOP_PER_PAIR_Q * pairs * nq / avg_s / 1e12
+        out[label] = {"atoms": n, "q_bins": nq, "steps": steps, "ms_per_step": wall * 1e3,
+                      "pair_q_per_s": pairs * nq / wall,
+                      "fused_avg_launch_ms": avg_s * 1e3, "achieved_tflops": achieved,
+                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(label)}
+        del g, f
+    return out
+
+
 def _first_at_or_below(rw, target):
     hit = np.nonzero(rw <= target)[0]
     return int(hit[0]) if hit.size else None
@@ -351,8 +406,9 @@ def kernel_suite(reps=3):
     F-only and pair stage on the 6,699-atom sphere (1,250 bins; all 22.4 M
     pairs), F-only and fused F + gradient on a 20,113-atom Au sphere (1,250 bins) and on
     the 20,000-atom Pt/Ni alloy (1,500 bins), the transform GEMM on its grad-G rows (3 N x 330 -> 4,000 r
-    points), contracted forces and learned-integrator forces on 64 chains of
-    5,000 atoms."""
+    points), contracted forces, the F-only kernel of an energy call (with the
+    steps it ran in each recurrence form) and learned-integrator forces on 64
+    chains of 5,000 atoms."""
     import torch
     from pyiid_amd import _lib, workloads
     from pyiid_amd.elasticscatter import ElasticScatter
@@ -363,13 +419,17 @@ def kernel_suite(reps=3):
     n = len(pos)
     pairs = n * (n - 1) // 2
 
-    def timed(ctx, kind, fn):
+    def timed(ctx, kind, fn, forms=None):
         fn()
         ctx.synchronize()
         ctx.enable_timing(True)
         for _ in range(reps):
             fn()
         ms, cnt = ctx.kernel_time(kind)
+        if forms is not None:
+            three, reinsch = ctx.fq_form_counts()
+            forms.update({"three_term_steps": three // reps, "reinsch_steps": reinsch // reps,
+                          "reinsch_share": reinsch / max(1, three + reinsch)})
         ctx.enable_timing(False)
         return ms / max(1, cnt)
 
@@ -454,6 +514,14 @@ def kernel_suite(reps=3):
         _lib.SLOT_PDF, scat.pdf_qbin, gt, _lib.POT_RW, 100.0, nat))
     kpairs = sum(len(x) * (len(x) - 1) // 2 for x in starts)
     out["pair_force_kernel"] = {"ms": ms, "pair_q_per_s": kpairs * nqp / (ms * 1e-3)}
+    # the F-only kernel of the same energy evaluation, with the (wave, j atom)
+    # steps it ran in each recurrence form
+    forms = {}
+    ms = timed(h.ctx, _lib.KERNEL_FQ, lambda: h.ctx.pdf_energy_forces(
+        _lib.SLOT_PDF, scat.pdf_qbin, gt, _lib.POT_RW, 100.0, nat, forces=False), forms)
+    out["fq_kernel_large"] = dict({"ms": ms, "chains": 64, "atoms_per_chain": 5000,
+                                   "q_bins": nqp, "pair_q_per_s": kpairs * nqp / (ms * 1e-3)},
+                                  **forms)
     h.train()
     h.forces()
     torch.cuda.synchronize()
@@ -487,6 +555,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-suite", action="store_true",
                     help="only run every hot kernel on its own workload (for profiling)")
+    ap.add_argument("--only-20k", action="store_true",
+                    help="only run the two 20k-atom F + grad workloads (for profiling)")
+    ap.add_argument("--skip-20k", action="store_true",
+                    help="leave the 20k-atom workloads out of the JSON line (for profiling)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="only run the 256-chain sampler timing (for tracing its graphs)")
     ap.add_argument("--hmc-proposals", type=int, default=16,
                     help="exact-HMC proposals of the sampler comparison (0 skips it)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -504,6 +578,12 @@ def main():
         return
     if args.kernel_suite:
         print(json.dumps(kernel_suite()))
+        return
+    if args.only_20k:
+        print(json.dumps(headline_20k()))
+        return
+    if args.only_batched:
+        print(json.dumps(batched_hmc()))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -576,13 +656,15 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": read_traffic(pairs * NQ),
+            "traffic": read_traffic(),
             "avg_launch_ms": avg_s * 1e3,
             "launches": fused_n,
             "flop_per_unit": FLOP_PER_PAIR_Q,
         },
     }
     ctx.close()
+    if not args.skip_20k:
+        out["headline_20k"] = headline_20k()
     if args.hmc_proposals > 0:
         out["hmc"] = hmc_comparison(args.hmc_proposals)
         out["batched_hmc"] = batched_hmc()

@@ -68,6 +68,12 @@ int pyiid_synchronize(pyiid_ctx *ctx);
  *       3 = pair-geometry kernel, 4 = contracted-force (unordered pair) kernel. */
 int pyiid_enable_timing(pyiid_ctx *ctx, int enable);
 int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launches);
+/* While timing is enabled, the F-only kernel counts its (wave, j atom) steps
+ * by recurrence form: the three-term form, or the Reinsch form that a wave
+ * takes when any of its pairs has a per-bin phase step within 0.2 of a
+ * multiple of pi.  Counts since timing was last enabled.  No counterpart in
+ * the reference. */
+int pyiid_fq_form_counts(pyiid_ctx *ctx, int64_t *three_term, int64_t *reinsch);
 /* Q bins per wave of the F-only kernel: 64, 72, 80, 88, 96 or 112 fixes the
  * launch shape; 0 (the default) lets the autotuner time the six once per
  * problem shape and keep the fastest.  The shapes differ only in how the bins are

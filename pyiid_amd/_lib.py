@@ -50,6 +50,7 @@ _SIGNATURES = [
     ("pyiid_synchronize", [_P]),
     ("pyiid_enable_timing", [_P, _I]),
     ("pyiid_kernel_time", [_P, _I, ctypes.POINTER(_D), ctypes.POINTER(_I64)]),
+    ("pyiid_fq_form_counts", [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     ("pyiid_set_fq_bins_per_wave", [_P, _I]),
     ("pyiid_set_grad_bins_per_wave", [_P, _I]),
     ("pyiid_set_autotune", [_P, _I]),
@@ -205,6 +206,13 @@ class Context:
         n = ctypes.c_int64()
         check(self.lib.pyiid_kernel_time(self.handle, kind, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def fq_form_counts(self):
+        """(three-term, Reinsch) (wave, j atom) steps of the F-only kernel
+        since timing was enabled."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.pyiid_fq_form_counts(self.handle, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     # --- structure ---------------------------------------------------------
     def set_positions(self, pos, flags=0):

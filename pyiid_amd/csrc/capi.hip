@@ -192,6 +192,7 @@ struct pyiid_ctx {
 
   // workspaces
   Buf gpart, fpart, fq64, fq32, g64, io_a, io_b, io_c, w, out5, u, uq, gobs, pfpart;
+  Buf formcnt;  // uint64 [2]: F-only steps in the three-term / Reinsch form (while timing)
 
   Hmc hmc;
   Sur sur;
@@ -588,12 +589,16 @@ int64_t min_chain_atoms(pyiid_ctx* c) {
 
 int tune_fq(pyiid_ctx* c, int slot, double qbin);
 
-hipError_t launch_fq_items(pyiid_ctx* c, const Table& t, double qbin) {
+// counted: add the launch's three-term / Reinsch steps to the context's
+// counts while timing is on (not for the tuner's trial launches)
+hipError_t launch_fq_items(pyiid_ctx* c, const Table& t, double qbin, bool counted = false) {
+  unsigned long long* forms =
+      counted && c->timing && !c->capturing ? c->formcnt.as<unsigned long long>() : nullptr;
   return launch_fq(c->stream, c->fq_m, c->compensated, c->pos.as<AtomRec>(),
                    c->fq_items.perm.as<int32_t>(),
                    t.tab.as<float>(), t.nq, static_cast<double>(static_cast<float>(qbin)),
                    c->fq_items.items.as<Item>(), c->fq_items.n_items, weighted_table(t),
-                   c->fpart.as<double>());
+                   c->fpart.as<double>(), forms);
 }
 
 // F(Q) of every chain (full grid) into c->fq64 [B][nq] (double) and c->fq32.
@@ -608,7 +613,7 @@ int compute_fq(pyiid_ctx* c, int slot, double qbin) {
   CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, c->fq_items.n_items) * nq));
   CHK(tune_fq(c, slot, qbin));
   Timed tm(c, 1);
-  HIPCHK(launch_fq_items(c, t, qbin));
+  HIPCHK(launch_fq_items(c, t, qbin, true));
   CHK(tm.done());
   // chains without pairs have empty ranges: their F is 0
   HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), c->fq_items.ranges.as<Range>(), B,
@@ -1471,6 +1476,22 @@ int pyiid_enable_timing(pyiid_ctx* c, int enable) {
     c->t_ms[k] = 0.0;
     c->t_n[k] = 0;
   }
+  if (c->timing) {
+    CHK(c->formcnt.reserve(2 * sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(c->formcnt.p, 0, 2 * sizeof(unsigned long long), c->stream));
+  }
+  return PYIID_OK;
+}
+
+int pyiid_fq_form_counts(pyiid_ctx* c, int64_t* three_term, int64_t* reinsch) {
+  CHK(check_ctx(c));
+  unsigned long long h[2] = {0, 0};
+  if (c->formcnt.p) {
+    HIPCHK(hipMemcpyAsync(h, c->formcnt.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (three_term) *three_term = static_cast<int64_t>(h[0]);
+  if (reinsch) *reinsch = static_cast<int64_t>(h[1]);
   return PYIID_OK;
 }
 

@@ -584,7 +584,7 @@ template <int M, bool WEIGHTED, bool COMP>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(COMP ? 2 : 3))) void
 fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
           const float* __restrict__ ftab, int nq, double dq, const Item* __restrict__ items,
-          double* __restrict__ fpart) {
+          double* __restrict__ fpart, unsigned long long* __restrict__ forms) {
   static_assert(M % 8 == 0 && M <= 128, "bins per wave");
   __shared__ __attribute__((aligned(32))) unsigned char smem[kFqSmemBytes];
   AtomRec* tile = reinterpret_cast<AtomRec*>(smem);
@@ -597,6 +597,9 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = it.i_base + lane;
+  // (wave, j atom) steps run in the three-term and in the Reinsch form
+  // (wave-uniform counts, added to forms[0], forms[1] at the end when given)
+  unsigned n_three = 0, n_reinsch = 0;
   // The block's kWaves Q chunks go to its waves in an order rotated by a hash
   // of the block index: in a Q group with fewer chunks than waves (the last,
   // when kWaves M does not divide nq) the working waves of the blocks that
@@ -674,6 +677,8 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
           const int o = k * 64 + lane;
           if (__all(geo_f[o] == 0.0f || geo_f[o + 4 * kGS] >= kChebEpsMin)) three |= 1u << k;
         }
+        n_three += __builtin_popcount(three);
+        n_reinsch += kk - __builtin_popcount(three);
         // Two j atoms at a time: two independent recurrences per bin keep the
         // wave issuing while one waits on its previous step.  An odd atom
         // left over runs alone after the loop.
@@ -719,6 +724,10 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
     const double hi = lane_reduce_f64<M, 64, M - 64>(acc, smem, wave, lane);
     if (lane < M - 64 && q0 + 64 + lane < nq)
       fpart[(size_t)blockIdx.x * nq + q0 + 64 + lane] = hi * weight(q0 + 64 + lane);
+  }
+  if (forms != nullptr && lane == 0) {
+    atomicAdd(&forms[0], static_cast<unsigned long long>(n_three));
+    atomicAdd(&forms[1], static_cast<unsigned long long>(n_reinsch));
   }
 }
 
@@ -973,43 +982,49 @@ __global__ __launch_bounds__(1024) void reduce_ranges_kernel(const double* __res
   if (out32) out32[(size_t)r * nq + q] = static_cast<float>(v);
 }
 
-// The same two bins per thread (8-byte loads), for even nq.
-template <typename T>
-__global__ void reduce_grad2_kernel(const float* __restrict__ gpart, int n_split, int64_t rows,
-                                    int nq, const double* __restrict__ gscale,
-                                    T* __restrict__ out) {
-  const int q = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
-  if (q >= nq) return;
-  const double g0 = gscale[q], g1 = gscale[q + 1];
-  const size_t total = static_cast<size_t>(rows) * nq;
-  for (int64_t row = blockIdx.y; row < rows; row += gridDim.y) {
-    const size_t e = static_cast<size_t>(row) * nq + q;
-    double a0 = 0.0, a1 = 0.0;
-    for (int s = 0; s < n_split; ++s) {
-      const float2 v = *reinterpret_cast<const float2*>(gpart + s * total + e);
-      a0 += static_cast<double>(v.x);
-      a1 += static_cast<double>(v.y);
+// out[e] = gscale[e mod nq] * sum_s gpart[s][e] in FP64 over the flat
+// element index e of the rows x nq planes (rows = (atom, component)); the
+// planes are summed in plane order.  Grid-strided; PAIR: two elements per
+// thread with 8-byte loads (the plane size is even, so every plane stays
+// 8-byte aligned).  The loads of four planes are issued before their adds,
+// so a thread keeps several HBM reads in flight.
+template <typename T, bool PAIR>
+__global__ __launch_bounds__(256) void reduce_grad_kernel(const float* __restrict__ gpart,
+                                                          int n_split, int64_t total, int nq,
+                                                          const double* __restrict__ gscale,
+                                                          T* __restrict__ out) {
+  constexpr int kPer = PAIR ? 2 : 1;
+  using V = typename std::conditional<PAIR, float2, float>::type;
+  const int64_t n_vec = total / kPer;
+  const V* planes = reinterpret_cast<const V*>(gpart);
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n_vec;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    double acc[kPer] = {};
+    auto add = [&](const V& x) {
+      if constexpr (PAIR) {
+        acc[0] += static_cast<double>(x.x);
+        acc[1] += static_cast<double>(x.y);
+      } else {
+        acc[0] += static_cast<double>(x);
+      }
+    };
+    int s = 0;
+    for (; s + 4 <= n_split; s += 4) {
+      const V x0 = planes[(s + 0) * n_vec + v], x1 = planes[(s + 1) * n_vec + v];
+      const V x2 = planes[(s + 2) * n_vec + v], x3 = planes[(s + 3) * n_vec + v];
+      add(x0);
+      add(x1);
+      add(x2);
+      add(x3);
     }
-    out[e] = static_cast<T>(a0 * g0);
-    out[e + 1] = static_cast<T>(a1 * g1);
-  }
-}
-
-// out[row][q] = gscale[q] * sum_s gpart[s][row][q] in FP64; rows are
-// (atom, component), n_split partial planes of rows x nq floats.
-template <typename T>
-__global__ void reduce_grad_kernel(const float* __restrict__ gpart, int n_split, int64_t rows,
-                                   int nq, const double* __restrict__ gscale, T* __restrict__ out) {
-  // x: bins of a row (coalesced), y: rows (atom, component), grid-strided
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nq) return;
-  const double g = gscale[q];
-  const size_t total = static_cast<size_t>(rows) * nq;
-  for (int64_t row = blockIdx.y; row < rows; row += gridDim.y) {
-    const size_t e = static_cast<size_t>(row) * nq + q;
-    double acc = 0.0;
-    for (int s = 0; s < n_split; ++s) acc += static_cast<double>(gpart[s * total + e]);
-    out[e] = static_cast<T>(acc * g);
+    for (; s < n_split; ++s) add(planes[s * n_vec + v]);
+    const int64_t e = v * kPer;
+    int q = static_cast<int>(e % nq);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      out[e + k] = static_cast<T>(acc[k] * gscale[q]);
+      if (++q == nq) q = 0;
+    }
   }
 }
 
@@ -1151,52 +1166,52 @@ hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double
 template <int M, bool WEIGHTED, bool COMP = false>
 static void launch_fq_m(hipStream_t st, const AtomRec* pos, const int32_t* perm,
                         const float* ftab, int nq, double dq, const Item* items, int n_items,
-                        double* fpart) {
+                        double* fpart, unsigned long long* forms) {
   dim3 grid(n_items, (nq + kWaves * M - 1) / (kWaves * M));
   hipLaunchKernelGGL((fq_kernel<M, WEIGHTED, COMP>), grid, dim3(64 * kWaves), 0, st, pos, perm,
-                     ftab, nq, dq, items, fpart);
+                     ftab, nq, dq, items, fpart, forms);
 }
 
 hipError_t launch_fq(hipStream_t st, int m, bool compensated, const AtomRec* pos,
                      const int32_t* perm, const float* ftab, int nq, double dq, const Item* items,
-                     int n_items, bool weighted, double* fpart) {
+                     int n_items, bool weighted, double* fpart, unsigned long long* forms) {
   if (n_items <= 0) return hipSuccess;
   if (compensated) {
     if (m != 64) return hipErrorInvalidValue;
     if (weighted)
-      launch_fq_m<64, true, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<64, true, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else
-      launch_fq_m<64, false, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<64, false, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     return hipGetLastError();
   }
   if (weighted) {
     if (m == 64)
-      launch_fq_m<64, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<64, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 72)
-      launch_fq_m<72, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<72, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 80)
-      launch_fq_m<80, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<80, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 88)
-      launch_fq_m<88, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<88, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 96)
-      launch_fq_m<96, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<96, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 112)
-      launch_fq_m<112, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<112, true>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else
       return hipErrorInvalidValue;
   } else {
     if (m == 64)
-      launch_fq_m<64, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<64, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 72)
-      launch_fq_m<72, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<72, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 80)
-      launch_fq_m<80, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<80, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 88)
-      launch_fq_m<88, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<88, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 96)
-      launch_fq_m<96, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<96, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else if (m == 112)
-      launch_fq_m<112, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart);
+      launch_fq_m<112, false>(st, pos, perm, ftab, nq, dq, items, n_items, fpart, forms);
     else
       return hipErrorInvalidValue;
   }
@@ -1214,26 +1229,27 @@ hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range*
 
 hipError_t launch_reduce_grad(hipStream_t st, const float* gpart, int n_split, int64_t n, int nq,
                               const double* gscale, void* out, bool out_f64) {
-  const int64_t rows = n * 3;
-  if (rows <= 0 || nq <= 0) return hipSuccess;
-  const unsigned grid_y = static_cast<unsigned>(std::min<int64_t>(rows, 65535));
-  if (nq % 2 == 0 && reinterpret_cast<uintptr_t>(gpart) % 8 == 0) {
-    const dim3 grid2((nq / 2 + 255) / 256, grid_y);
+  const int64_t total = n * 3 * nq;
+  if (total <= 0) return hipSuccess;
+  const bool pair = total % 2 == 0 && reinterpret_cast<uintptr_t>(gpart) % 8 == 0;
+  const int64_t n_vec = pair ? total / 2 : total;
+  // grid-strided over at most 32 blocks per CU (256 CUs)
+  const unsigned blocks = static_cast<unsigned>(std::min<int64_t>((n_vec + 255) / 256, 256 * 32));
+  if (pair) {
     if (out_f64)
-      hipLaunchKernelGGL(reduce_grad2_kernel<double>, grid2, dim3(256), 0, st, gpart, n_split,
-                         rows, nq, gscale, static_cast<double*>(out));
+      hipLaunchKernelGGL((reduce_grad_kernel<double, true>), dim3(blocks), dim3(256), 0, st, gpart,
+                         n_split, total, nq, gscale, static_cast<double*>(out));
     else
-      hipLaunchKernelGGL(reduce_grad2_kernel<float>, grid2, dim3(256), 0, st, gpart, n_split,
-                         rows, nq, gscale, static_cast<float*>(out));
-    return hipGetLastError();
+      hipLaunchKernelGGL((reduce_grad_kernel<float, true>), dim3(blocks), dim3(256), 0, st, gpart,
+                         n_split, total, nq, gscale, static_cast<float*>(out));
+  } else {
+    if (out_f64)
+      hipLaunchKernelGGL((reduce_grad_kernel<double, false>), dim3(blocks), dim3(256), 0, st,
+                         gpart, n_split, total, nq, gscale, static_cast<double*>(out));
+    else
+      hipLaunchKernelGGL((reduce_grad_kernel<float, false>), dim3(blocks), dim3(256), 0, st,
+                         gpart, n_split, total, nq, gscale, static_cast<float*>(out));
   }
-  const dim3 grid((nq + 255) / 256, grid_y);
-  if (out_f64)
-    hipLaunchKernelGGL(reduce_grad_kernel<double>, grid, dim3(256), 0, st, gpart, n_split, rows,
-                       nq, gscale, static_cast<double*>(out));
-  else
-    hipLaunchKernelGGL(reduce_grad_kernel<float>, grid, dim3(256), 0, st, gpart, n_split, rows,
-                       nq, gscale, static_cast<float*>(out));
   return hipGetLastError();
 }
 

@@ -66,10 +66,11 @@ hipError_t launch_uq(hipStream_t st, const float* u, int n_chain, int nq, double
 // Unordered-pair F only: fpart[item][q] (double), m Q bins per wave (64, 72,
 // 80, 88, 96 or 112; compensated sums: 64 only).  Item indices are positions
 // in perm; weighted: item.split is the element of the item's i tile and
-// item.pad that of its j range.
+// item.pad that of its j range.  forms (nullable): forms[0], forms[1] gain
+// the (wave, j atom) steps run in the three-term and in the Reinsch form.
 hipError_t launch_fq(hipStream_t st, int m, bool compensated, const AtomRec* pos,
                      const int32_t* perm, const float* ftab, int nq, double dq, const Item* items,
-                     int n_items, bool weighted, double* fpart);
+                     int n_items, bool weighted, double* fpart, unsigned long long* forms);
 // out[r][q] = scale[r][q] * sum_{p in ranges[r]} part[p][q], r < n_rows (FP64)
 hipError_t launch_reduce_ranges(hipStream_t st, const double* part, const Range* ranges,
                                 int n_rows, int nq, const double* scale, double* out_f64,

@@ -333,7 +333,7 @@ extern "C" int pyiid_fq_noalloc(int device, void* stream, const float* pos, cons
     const int base = p * per_pass;
     const int cnt = static_cast<int>(std::min<int64_t>(per_pass, item_cap - base));
     NA_HIP(launch_fq(st, kFqM, false, prec, perm, table, n_q, dq, items + base, cnt, weighted,
-                     part));
+                     part, nullptr));
     hipLaunchKernelGGL(accumulate_kernel, dim3((n_q + 63) / 64), dim3(1024), 0, st, s, acc, part,
                        base, per_pass, n_q, p == 0 ? 1 : 0, p == passes - 1 ? 1 : 0, fq);
     NA_HIP(hipGetLastError());
