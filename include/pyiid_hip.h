@@ -254,16 +254,19 @@ int pyiid_sur_train(pyiid_ctx *ctx, double forget, double ridge, double *r2, int
 /* n_proposals proposals driven by the surrogate: forces from the network,
  * velocity Verlet and momenta as in pyiid_hmc_run, acceptance with the exact
  * energy of the end point.  A training step runs before proposal p whenever
- * p % train_every == 0 (train_every = 0: only if no fit exists).  When the
- * mean r2 of a training step is below min_r2, the proposals until the next
- * step use exact forces (pyiid_hmc_run's proposal).  energies / accepted /
+ * p % train_every == 0 (train_every = 0: only if no fit exists).  The
+ * proposals until the next step use exact forces (pyiid_hmc_run's proposal)
+ * when the mean r2 of the training step is below min_r2, or when the
+ * learned-force proposals since the previous step were accepted less often
+ * than min_acceptance (0 disables the test; after an exact block the learned
+ * forces are tried again).  energies / accepted /
  * exact_mode (per proposal) / times_ms as in pyiid_hmc_run; violations
  * (n_chain int32, nullable): proposals rejected for leaving the neighbour
  * lists' range, counted since the layout was set. */
 int pyiid_sur_hmc_run(pyiid_ctx *ctx, int n_proposals, int n_leapfrog, double step, double beta,
                       uint64_t seed, int use_graph, int train_every, double forget, double ridge,
-                      double min_r2, double *energies, int32_t *accepted, int32_t *exact_mode,
-                      double *times_ms, int32_t *violations, int flags);
+                      double min_r2, double min_acceptance, double *energies, int32_t *accepted,
+                      int32_t *exact_mode, double *times_ms, int32_t *violations, int flags);
 
 /* ---- F(Q) without device allocation ------------------------------------ */
 #define PYIID_NOALLOC_MAX_ATOMS 65536

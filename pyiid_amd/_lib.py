@@ -79,7 +79,7 @@ _SIGNATURES = [
     ("pyiid_sur_forces", [_P, _P, _P, _I]),
     ("pyiid_sur_features", [_P, _P, _I]),
     ("pyiid_sur_train", [_P, _D, _D, _P, _I]),
-    ("pyiid_sur_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _I, _D, _D, _D, _P, _P, _P,
+    ("pyiid_sur_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _I, _D, _D, _D, _D, _P, _P, _P,
                            _P, _P, _I]),
     ("pyiid_fq_noalloc", [_I, _P, _P, _P, _I64, _P, _I32, _I32, _D, _P]),
 ]
@@ -391,7 +391,7 @@ class Context:
         return r2
 
     def sur_hmc_run(self, n_proposals, n_leapfrog, step, beta, seed, n_chain, use_graph=True,
-                    train_every=0, forget=0.5, ridge=1e-6, min_r2=-np.inf):
+                    train_every=0, forget=0.5, ridge=1e-6, min_r2=-np.inf, min_acceptance=0.0):
         e = np.empty((n_proposals, n_chain), np.float64)
         a = np.empty((n_proposals, n_chain), np.int32)
         m = np.empty(n_proposals, np.int32)
@@ -400,7 +400,8 @@ class Context:
         check(self.lib.pyiid_sur_hmc_run(
             self.handle, int(n_proposals), int(n_leapfrog), float(step), float(beta),
             int(seed) & (2 ** 64 - 1), 1 if use_graph else 0, int(train_every), float(forget),
-            float(ridge), float(min_r2), _ptr(e), _ptr(a), _ptr(m), _ptr(t), _ptr(v), 0))
+            float(ridge), float(min_r2), float(min_acceptance), _ptr(e), _ptr(a), _ptr(m), _ptr(t),
+            _ptr(v), 0))
         return {"energies": e, "accepted": a, "exact_mode": m, "times_ms": t, "violations": v}
 
     def fq_energy_forces(self, slot, qbin, q_first, gobs, potential, conv, n_atoms, forces=True):

@@ -2118,8 +2118,8 @@ int pyiid_sur_train(pyiid_ctx* c, double forget, double ridge, double* r2, int f
 
 int pyiid_sur_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step, double beta,
                       uint64_t seed, int use_graph, int train_every, double forget, double ridge,
-                      double min_r2, double* energies, int32_t* accepted, int32_t* exact_mode,
-                      double* times_ms, int32_t* violations, int flags) {
+                      double min_r2, double min_acceptance, double* energies, int32_t* accepted,
+                      int32_t* exact_mode, double* times_ms, int32_t* violations, int flags) {
   CHK(sur_ready(c));
   Hmc& h = c->hmc;
   Sur& s = c->sur;
@@ -2135,12 +2135,27 @@ int pyiid_sur_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step
   CHK(prepare_history(c, n_proposals));
   RunClock clock(c, times_ms != nullptr);
   bool exact_block = false;
+  int block_start = 0;
+  std::vector<int32_t> acc_h;
   for (int p = 0; p < n_proposals; ++p) {
     if (!s.w3_set || (train_every > 0 && p % train_every == 0)) {
+      // acceptance of the learned-force block that just ended
+      bool low_acc = false;
+      if (min_acceptance > 0.0 && p > block_start && !exact_block) {
+        acc_h.resize(static_cast<size_t>(p - block_start) * B);
+        HIPCHK(hipMemcpyAsync(acc_h.data(), h.acchist.as<int32_t>() + (size_t)block_start * B,
+                              sizeof(int32_t) * acc_h.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int64_t n_acc = 0;
+        for (int32_t a : acc_h) n_acc += a;
+        low_acc = static_cast<double>(n_acc) < min_acceptance * static_cast<double>(acc_h.size());
+      }
       CHK(sur_train_step(c, forget, ridge));
       double mean = 0.0;
       for (int b = 0; b < B; ++b) mean += s.r2[b];
-      exact_block = mean / B < min_r2;  // poor fit: exact forces until the next fit
+      // poor fit, or learned proposals mostly rejected: exact forces until the next fit
+      exact_block = mean / B < min_r2 || low_acc;
+      block_start = p;
     }
     if (exact_block) CHK(refresh_exact(c));
     CHK(run_proposal(c, !exact_block, use_graph != 0, n_leapfrog, step, beta, seed));

@@ -213,11 +213,14 @@ class SurrogateHMC(DeviceHMC):
         return self.ctx.sur_train(self.B, forget, ridge)
 
     def run(self, n_proposals, n_leapfrog=10, step=2e-3, beta=1.0, seed=0, use_graph=True,
-            train_every=10, forget=0.5, ridge=1e-6, min_r2=-np.inf):
+            train_every=10, forget=0.5, ridge=1e-6, min_r2=-np.inf, min_acceptance=0.0):
         """Returns a dict of energies / accepted (n_proposals, B), exact_mode
-        (proposals run with exact forces), times_ms, violations."""
+        (proposals run with exact forces), times_ms, violations.  A block of
+        train_every proposals runs with exact forces when the fit before it
+        has mean r2 < min_r2 or the learned-force block before it accepted
+        less often than min_acceptance."""
         return self.ctx.sur_hmc_run(n_proposals, n_leapfrog, step, beta, seed, self.B, use_graph,
-                                    train_every, forget, ridge, min_r2)
+                                    train_every, forget, ridge, min_r2, min_acceptance)
 
     def run_exact(self, n_proposals, n_leapfrog=10, step=2e-3, beta=1.0, seed=0, use_graph=True,
                   times=False):

@@ -225,3 +225,19 @@ def test_neighbour_capacity_is_checked():
     chains = _chains([40], seed=10)
     with pytest.raises(ValueError, match="neighbours"):
         _sampler(chains, max_nbr=8)
+
+
+def test_low_acceptance_switches_the_next_block_to_exact_forces():
+    """min_acceptance: a learned-force block accepted less often than the bar
+    is followed by a block of exact-force proposals, after which the learned
+    forces are tried again.  A bar above 1 fails every learned block, so the
+    blocks alternate; a bar of 0 never switches."""
+    chains = _chains([24, 30], seed=11)
+    hmc, _, _, _ = _sampler(chains, conv=50.0)
+    out = hmc.run(30, n_leapfrog=3, step=0.001, beta=5.0, seed=2, train_every=10,
+                  min_acceptance=1.01)
+    np.testing.assert_array_equal(out["exact_mode"], [0] * 10 + [1] * 10 + [0] * 10)
+    out = hmc.run(20, n_leapfrog=3, step=0.001, beta=5.0, seed=3, train_every=10,
+                  min_acceptance=0.0)
+    np.testing.assert_array_equal(out["exact_mode"], [0] * 20)
+    hmc.close()
