@@ -90,6 +90,23 @@ def compact_scatter(atoms, name):
     return table, index.reshape(-1).astype(np.int32)
 
 
+def _host_out(shape, dtype):
+    """A new host array for a result copied from the device.  With torch and a
+    device present it is page-locked memory from torch's caching host
+    allocator, so the device-to-host copy runs at DMA speed instead of through
+    pageable staging (the 200 MB float64 gradient of a 6,699-atom particle);
+    otherwise plain memory.  Every call returns a new array, as the reference's
+    wrappers do."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            tdt = torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
+            return torch.empty(shape, dtype=tdt, pin_memory=True).numpy()
+    except (ImportError, RuntimeError):
+        pass
+    return np.empty(shape, dtype)
+
+
 def _slot(sum_type):
     return _lib.SLOT_FQ if sum_type == "fq" else _lib.SLOT_PDF
 
@@ -119,7 +136,8 @@ def wrap_fq(atoms, qbin=.1, sum_type="fq"):
 def wrap_fq_grad(atoms, qbin=.1, sum_type="fq"):
     """-1/2 dF/dq as an (N, 3, Q) float64 array (the reference convention)."""
     with loaded(atoms, sum_type) as (st, slot, nq):
-        return st.ctx.grad_fq(slot, qbin, len(atoms), nq, f64=True)
+        out = _host_out((len(atoms), 3, nq), np.float64)
+        return st.ctx.grad_fq(slot, qbin, len(atoms), nq, out=out, f64=True)
 
 
 def grad_pdf(grad_fq, rstep, qstep, rgrid, qmin):
@@ -128,7 +146,8 @@ def grad_pdf(grad_fq, rstep, qstep, rgrid, qmin):
     st = state()
     with st.lock:
         st.load_transform(rstep, qstep, rgrid, qmin, grad_fq.shape[-1])
-        return st.ctx.grad_pdf(grad_fq, len(rgrid))
+        out = _host_out(tuple(grad_fq.shape[:-1]) + (len(rgrid),), np.float64)
+        return st.ctx.grad_pdf(grad_fq, len(rgrid), out=out)
 
 
 def pdf_at_qmin(fq, rstep, qstep, rgrid, qmin):

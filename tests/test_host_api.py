@@ -144,9 +144,11 @@ def test_shared_context_serves_each_thread_its_own_structure(monkeypatch):
             time.sleep(0.001)
             return np.full(nq, self.pos.sum(), np.float32)
 
-        def grad_fq(self, slot, qbin, n, nq, f64=False):
+        def grad_fq(self, slot, qbin, n, nq, out=None, f64=False):
             time.sleep(0.001)
-            return np.full((n, 3, nq), self.pos.sum())
+            out = np.empty((n, 3, nq)) if out is None else out
+            out[...] = self.pos.sum()
+            return out
 
     monkeypatch.setattr(processor._lib, "Context", Recording)
     monkeypatch.setattr(processor, "_ctx", {})
@@ -155,17 +157,21 @@ def test_shared_context_serves_each_thread_its_own_structure(monkeypatch):
         a = Atoms("Au%d" % (5 + k), np.random.RandomState(k).random_sample((5 + k, 3)) * 5)
         a.set_array("F(Q) scatter", np.ones((len(a), 30), np.float32))
         structs.append(a)
-    wrong = []
+    wrong, errors = [], []
 
     def work(a):
         want = np.float32(np.asarray(a.get_positions(), np.float32).sum())
-        for _ in range(40):
-            if processor.wrap_fq(a)[0] != want or processor.wrap_fq_grad(a)[0, 0, 0] != want:
-                wrong.append(len(a))
+        try:
+            for _ in range(40):
+                if processor.wrap_fq(a)[0] != want or processor.wrap_fq_grad(a)[0, 0, 0] != want:
+                    wrong.append(len(a))
+        except Exception as exc:  # a failure in a thread must fail the test
+            errors.append(repr(exc))
 
     threads = [threading.Thread(target=work, args=(a,)) for a in structs]
     for t in threads:
         t.start()
     for t in threads:
         t.join()
+    assert errors == []
     assert wrong == []

@@ -109,3 +109,22 @@ def test_noalloc_rejects_sizes_beyond_its_limits():
         rc = lib.pyiid_fq_noalloc(0, None, p, None, n, p, n_elem, nq, qbin, p)
         assert rc == -1, (n, n_elem, nq, qbin)
         assert b"limit" in lib.pyiid_last_error() or qbin == 0.0
+
+
+@pytest.mark.gpu
+def test_noalloc_with_no_pair_gives_zero():
+    """No atoms, or one: F is 0 in every bin (no pair, na = 0 -> nan_to_num)."""
+    import torch
+    lib = _lib.load()
+    table, _, _ = element_table(np.array([79]), 0.1, 250)
+    table_d = torch.from_numpy(table).cuda()
+    stream = torch.cuda.current_stream().cuda_stream
+    for n in (0, 1):
+        pos_d = torch.zeros((n, 3), dtype=torch.float32, device="cuda") + 1.5
+        out_d = torch.full((250,), float("nan"), dtype=torch.float32, device="cuda")
+        _lib.check(lib.pyiid_fq_noalloc(0, ctypes.c_void_p(stream),
+                                        ctypes.c_void_p(pos_d.data_ptr() if n else 0), None, n,
+                                        ctypes.c_void_p(table_d.data_ptr()), 1, 250, 0.1,
+                                        ctypes.c_void_p(out_d.data_ptr())))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out_d.cpu().numpy(), np.zeros(250, np.float32))
