@@ -274,16 +274,39 @@ __global__ void gemv_rows_kernel(const double* __restrict__ x, int64_t rows, int
   if (lane == 0) y[row] = acc;
 }
 
-// u[b][q] = gscale[b][q] * sum_t T[t][q] w[b][t]  (FP64 accumulation; T float)
-__global__ void adjoint_kernel(const float* __restrict__ t, int n_r, int nq,
-                               const double* __restrict__ w, const double* __restrict__ gscale,
-                               float* __restrict__ u) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// u[b][q] = gscale[b][q] * sum_t T[t][q] w[b][t]  (FP64 accumulation; T float).
+// A block takes 64 bins of one chain (lane = bin, coalesced rows of T) with
+// kAdjWaves waves, each summing a contiguous slice of the rows into four
+// independent accumulators so that its loads overlap; the waves' partials
+// are added in wave order through LDS (deterministic).
+constexpr int kAdjWaves = 16;
+__global__ __launch_bounds__(64 * kAdjWaves) void adjoint_kernel(
+    const float* __restrict__ t, int n_r, int nq, const double* __restrict__ w,
+    const double* __restrict__ gscale, float* __restrict__ u) {
+  __shared__ double part[kAdjWaves][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int q = blockIdx.x * 64 + lane;
   const int b = blockIdx.y;
-  if (q >= nq) return;
   const double* wb = w + (size_t)b * n_r;
+  const int per = (n_r + kAdjWaves - 1) / kAdjWaves;
+  const int r0 = wave * per, r1 = min(n_r, r0 + per);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (q < nq) {
+    int r = r0;
+    for (; r + 4 <= r1; r += 4) {
+      a0 += static_cast<double>(t[(size_t)r * nq + q]) * wb[r];
+      a1 += static_cast<double>(t[(size_t)(r + 1) * nq + q]) * wb[r + 1];
+      a2 += static_cast<double>(t[(size_t)(r + 2) * nq + q]) * wb[r + 2];
+      a3 += static_cast<double>(t[(size_t)(r + 3) * nq + q]) * wb[r + 3];
+    }
+    for (; r < r1; ++r) a0 += static_cast<double>(t[(size_t)r * nq + q]) * wb[r];
+  }
+  part[wave][lane] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  if (wave != 0 || q >= nq) return;
   double acc = 0.0;
-  for (int r = 0; r < n_r; ++r) acc += static_cast<double>(t[(size_t)r * nq + q]) * wb[r];
+#pragma unroll
+  for (int k = 0; k < kAdjWaves; ++k) acc += part[k][lane];
   u[(size_t)b * nq + q] = static_cast<float>(acc * gscale[(size_t)b * nq + q]);
 }
 
@@ -358,8 +381,8 @@ hipError_t launch_gemv_rows(hipStream_t st, const double* x, int64_t rows, int n
 
 hipError_t launch_adjoint(hipStream_t st, const float* t, int n_r, int nq, const double* w,
                           const double* gscale, int n_chain, float* u) {
-  hipLaunchKernelGGL(adjoint_kernel, dim3((nq + 255) / 256, n_chain), dim3(256), 0, st, t, n_r, nq,
-                     w, gscale, u);
+  hipLaunchKernelGGL(adjoint_kernel, dim3((nq + 63) / 64, n_chain), dim3(64 * kAdjWaves), 0, st, t,
+                     n_r, nq, w, gscale, u);
   return hipGetLastError();
 }
 
