@@ -80,14 +80,30 @@ int pyiid_fq_form_counts(pyiid_ctx *ctx, int64_t *three_term, int64_t *reinsch);
  * grouped, so F agrees between them to FP32 rounding.  No counterpart in the
  * reference (its kernels have no launch-shape choice). */
 int pyiid_set_fq_bins_per_wave(pyiid_ctx *ctx, int m);
-/* Q bins per wave of the fused F + gradient kernel: 16 or 32 fixes the launch
- * shape (with the default j split); 0 (the default) lets the autotuner time
- * both, with three split sizes each, once per problem shape.  The shapes differ
- * only in how the bins are grouped, so the results agree to FP32 rounding.
- * No counterpart in the reference (its kernels have no launch-shape choice). */
+/* Q bins per wave of the ordered-pair F + gradient kernel: 16 or 32 runs
+ * pyiid_grad_fq on that kernel with that launch shape (and the default j
+ * split); 0 (the default) lets the autotuner choose between the symmetric
+ * (unordered-pair) kernel and the ordered one, and time the ordered one's two
+ * shapes with three split sizes each, once per problem shape.  Kernels and
+ * shapes differ only in how the sums are grouped, so the results agree to
+ * FP32 rounding.  No counterpart in the reference (its kernels have no
+ * launch-shape choice). */
 int pyiid_set_grad_bins_per_wave(pyiid_ctx *ctx, int m);
-/* Autotuning of the Debye kernels' launch shapes (fused kernel bins per wave
- * and j split, F-only bins per wave, contracted-force Q groups), on by
+/* Tiles of 64 atoms per I group and per J range of a work item of the
+ * symmetric F + gradient kernel: 4, 8, 16 or 32 runs pyiid_grad_fq on that
+ * kernel with that grouping (unless compensated sums are on or the ordered
+ * kernel's bins per wave are fixed); 0 (the default) leaves the choice to the
+ * autotuner.  Larger groups write fewer partial-sum slots (about 3 nq 64 x 4
+ * bytes each, tiles^2 / k of them) and give fewer, longer blocks.  No
+ * counterpart in the reference. */
+int pyiid_set_grad_tiles_per_item(pyiid_ctx *ctx, int k);
+/* Q bins per wave of the symmetric F + gradient kernel: 16 (8 waves per
+ * block) or 12 (12 waves per block) fixes the launch shape; 0 (the default)
+ * lets the autotuner time both.  The results agree to FP32 rounding.  No
+ * counterpart in the reference. */
+int pyiid_set_grad_sym_bins_per_wave(pyiid_ctx *ctx, int m);
+/* Autotuning of the Debye kernels' launch shapes (F + gradient kernel and its
+ * grouping, F-only bins per wave, contracted-force Q groups), on by
  * default: the first call for a problem shape times the candidates and keeps
  * the fastest.  Off (enable = 0): every call uses fixed default shapes, so
  * results are the same bit for bit from run to run and context to context;

@@ -53,6 +53,8 @@ _SIGNATURES = [
     ("pyiid_fq_form_counts", [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     ("pyiid_set_fq_bins_per_wave", [_P, _I]),
     ("pyiid_set_grad_bins_per_wave", [_P, _I]),
+    ("pyiid_set_grad_tiles_per_item", [_P, _I]),
+    ("pyiid_set_grad_sym_bins_per_wave", [_P, _I]),
     ("pyiid_set_autotune", [_P, _I]),
     ("pyiid_set_precision", [_P, _I]),
     ("pyiid_set_positions", [_P, _P, _I64, _I]),
@@ -200,6 +202,14 @@ class Context:
     def set_grad_bins_per_wave(self, m):
         """Fix the fused F + gradient kernel's Q bins per wave (16, 32), or 0 to autotune."""
         check(self.lib.pyiid_set_grad_bins_per_wave(self.handle, int(m)))
+
+    def set_grad_tiles_per_item(self, k):
+        """Fix the symmetric F + gradient kernel's tiles per item (4, 8, 16, 32), or 0 to autotune."""
+        check(self.lib.pyiid_set_grad_tiles_per_item(self.handle, int(k)))
+
+    def set_grad_sym_bins_per_wave(self, m):
+        """Fix the symmetric F + gradient kernel's Q bins per wave (12, 16), or 0 to autotune."""
+        check(self.lib.pyiid_set_grad_sym_bins_per_wave(self.handle, int(m)))
 
     def kernel_time(self, kind):
         ms = ctypes.c_double()

@@ -110,6 +110,17 @@ struct ItemSet {
   int key = -1;
 };
 
+// Work list of the symmetric F + gradient kernel: tiles of at most 64
+// positions in order_atoms order, items (I group x J range of k tiles each)
+// in decreasing order of pair count, and every tile's slot list.
+struct SymSet {
+  Buf items, tiles, entries, tile_el, perm, franges;
+  int n_items = 0, n_tiles = 0;
+  int64_t n_slots = 0;
+  uint64_t stamp = ~0ull;
+  int key = -1;
+};
+
 // One captured proposal and the parameters it was captured with.
 struct GraphSlot {
   hipGraph_t graph = nullptr;
@@ -176,6 +187,7 @@ struct pyiid_ctx {
   ItemSet fq_items;     // unordered pairs, ranges per chain
   ItemSet fused_items;  // ordered pairs with j splits, ranges per i tile
   ItemSet pair_items;   // tile pairs (I, J <= I) of the contracted-force kernel
+  SymSet sym;           // unordered tile pairs of the symmetric F + gradient kernel
   Buf pair_entries;     // (work, side) lists per tile
   int pf_groups = 1, pf_span = 32;
 
@@ -185,6 +197,10 @@ struct pyiid_ctx {
   int fq_m_fixed = 0;                      // set by pyiid_set_fq_bins_per_wave (0: autotuned)
   int grad_m_fixed = 0;                    // set by pyiid_set_grad_bins_per_wave (0: autotuned)
   int64_t fused_waves = kDefaultFusedWaves;  // target wave count that sets the fused j split
+  int sym_k = 0;  // tiles per I group / J range of the symmetric kernel (0: ordered kernel)
+  int sym_k_fixed = 0;  // set by pyiid_set_grad_tiles_per_item (0: autotuned)
+  int sym_m = 16;       // Q bins per wave of the symmetric kernel (16 or 12)
+  int sym_m_fixed = 0;  // set by pyiid_set_grad_sym_bins_per_wave (0: autotuned)
   int64_t pf_waves = kDefaultPfWaves;        // target wave count that sets the force Q groups
   bool autotune = true;
   bool compensated = false;  // compensated FP32 sums in the F and F + gradient kernels
@@ -531,6 +547,159 @@ int ensure_fused_items(pyiid_ctx* c, int nq, bool weighted) {
   return PYIID_OK;
 }
 
+// Slot buffer limit of the symmetric kernel (8 GiB of the 288 GB); a shape
+// that needs more for every group size runs the ordered kernel.
+constexpr size_t kSymSlotBudget = static_cast<size_t>(8) << 30;
+// Tiles per I group / J range tried by the autotuner.  A J range of at most
+// 32 tiles keeps every FP32 run of the gradient sums within kMaxJRun atoms.
+constexpr int kSymGroups[] = {4, 8, 16, 32};
+
+struct SymPlan {
+  std::vector<int32_t> perm;
+  std::vector<Range> tiles;  // {slot list begin, end, position begin, end}
+  std::vector<int> tile_el;
+  std::vector<int> chain_tiles{0};  // tiles of chain b: [chain_tiles[b], chain_tiles[b + 1])
+  std::vector<SymItem> items;
+  std::vector<int> entries;
+  int64_t n_slots = 0;
+};
+
+// Tiles of the symmetric kernel: runs of at most 64 positions of one chain
+// (and, for a weighted table, of one element) in order_atoms order.
+int plan_sym_tiles(pyiid_ctx* c, bool weighted, SymPlan& pl) {
+  std::vector<int64_t> seg_end;
+  CHK(order_atoms(c, weighted, pl.perm, seg_end));
+  auto el = [&](int64_t p) { return weighted && !c->elem_h.empty() ? c->elem_h[pl.perm[p]] : 0; };
+  pl.tiles.clear();
+  pl.tile_el.clear();
+  pl.chain_tiles.assign(1, 0);
+  for (int b = 0; b < c->n_chain; ++b) {
+    for (int64_t p = c->off[b]; p < c->off[b + 1];) {
+      const int64_t q = std::min(p + 64, seg_end[p]);
+      pl.tiles.push_back(Range{0, 0, static_cast<int>(p), static_cast<int>(q)});
+      pl.tile_el.push_back(el(p));
+      p = q;
+    }
+    pl.chain_tiles.push_back(static_cast<int>(pl.tiles.size()));
+  }
+  return PYIID_OK;
+}
+
+// Items for k tiles per I group and per J range (J ranges also end at element
+// changes), their slots, and every tile's slot list.  Items are ordered by
+// decreasing pair count so that the longest start first.
+int plan_sym_items(SymPlan& pl, int k) {
+  pl.items.clear();
+  pl.entries.clear();
+  std::vector<double> cost;
+  std::vector<std::vector<int>> per_tile(pl.tiles.size());
+  int64_t ns = 0;
+  for (size_t b = 0; b + 1 < pl.chain_tiles.size(); ++b) {
+    const int t0 = pl.chain_tiles[b], t1 = pl.chain_tiles[b + 1];
+    std::vector<std::pair<int, int>> jr;
+    for (int t = t0; t < t1;) {
+      int u = t + 1;
+      while (u < t1 && u - t < k && pl.tile_el[u] == pl.tile_el[t]) ++u;
+      jr.emplace_back(t, u);
+      t = u;
+    }
+    for (int g0 = t0; g0 < t1; g0 += k) {
+      const int g1 = std::min(t1, g0 + k);
+      for (const auto& r : jr) {
+        if (r.first >= g1) break;
+        if (ns + 2 * k > INT32_MAX) return fail(PYIID_E_INVALID, "too many atom pairs");
+        SymItem it{g0, g1, r.first, r.second, 0, 0, 0, 0};
+        const int i_first = std::max(g0, r.first);
+        it.islot0 = static_cast<int>(ns);
+        double w = 0.0;
+        for (int ti = i_first; ti < g1; ++ti) {
+          per_tile[ti].push_back(static_cast<int>(ns++));
+          w += std::min(r.second, ti + 1) - r.first - (ti < r.second ? 0.5 : 0.0);
+        }
+        it.jslot0 = static_cast<int>(ns);
+        for (int tj = r.first; tj < std::min(r.second, g1); ++tj)
+          per_tile[tj].push_back(static_cast<int>(ns++));
+        pl.items.push_back(it);
+        cost.push_back(w);
+      }
+    }
+  }
+  std::vector<int> ord(pl.items.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = static_cast<int>(i);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<SymItem> sorted(pl.items.size());
+  for (size_t i = 0; i < ord.size(); ++i) sorted[i] = pl.items[ord[i]];
+  pl.items.swap(sorted);
+  for (size_t t = 0; t < pl.tiles.size(); ++t) {
+    pl.tiles[t].begin = static_cast<int>(pl.entries.size());
+    pl.entries.insert(pl.entries.end(), per_tile[t].begin(), per_tile[t].end());
+    pl.tiles[t].end = static_cast<int>(pl.entries.size());
+  }
+  pl.n_slots = ns;
+  return PYIID_OK;
+}
+
+size_t sym_slot_bytes(int64_t n_slots, int nq) {
+  return sizeof(float) * static_cast<size_t>(std::max<int64_t>(1, n_slots)) * 3 * 64 *
+         static_cast<size_t>(std::max(1, nq));
+}
+
+template <typename T>
+int upload_vec(Buf& b, const std::vector<T>& v) {
+  CHK(b.reserve(sizeof(T) * std::max<size_t>(1, v.size())));
+  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return PYIID_OK;
+}
+
+int ensure_sym_items(pyiid_ctx* c, bool weighted, int k) {
+  SymSet& s = c->sym;
+  const int key = k * 2 + (weighted ? 1 : 0);
+  if (s.stamp == c->struct_stamp && s.key == key) return PYIID_OK;
+  if (c->capturing) return fail(PYIID_E_STATE, "structure changed during graph capture");
+  SymPlan pl;
+  CHK(plan_sym_tiles(c, weighted, pl));
+  CHK(plan_sym_items(pl, k));
+  CHK(upload_vec(s.items, pl.items));
+  CHK(upload_vec(s.tiles, pl.tiles));
+  CHK(upload_vec(s.entries, pl.entries));
+  CHK(upload_vec(s.tile_el, pl.tile_el));
+  CHK(upload_vec(s.perm, pl.perm));
+  const std::vector<Range> fr{Range{0, static_cast<int>(pl.items.size()), 0, static_cast<int>(c->n)}};
+  CHK(upload_vec(s.franges, fr));
+  s.n_items = static_cast<int>(pl.items.size());
+  s.n_tiles = static_cast<int>(pl.tiles.size());
+  s.n_slots = pl.n_slots;
+  s.key = key;
+  s.stamp = c->struct_stamp;
+  return PYIID_OK;
+}
+
+// Group sizes of the symmetric kernel whose slots fit kSymSlotBudget, with
+// their block counts at 128 bins per block.
+int sym_candidates(pyiid_ctx* c, int nq, bool weighted, std::vector<std::pair<int, int64_t>>& out) {
+  out.clear();
+  SymPlan pl;
+  CHK(plan_sym_tiles(c, weighted, pl));
+  const int64_t n_qg = (nq + 127) / 128;
+  for (int k : kSymGroups) {
+    CHK(plan_sym_items(pl, k));
+    if (sym_slot_bytes(pl.n_slots, nq) > kSymSlotBudget) continue;
+    out.emplace_back(k, static_cast<int64_t>(pl.items.size()) * n_qg);
+  }
+  return PYIID_OK;
+}
+
+// Group size without the autotuner: the largest with at least 4 blocks per
+// CU (256 CUs), else the smallest that fits; 0 (ordered kernel) if none fits.
+int default_sym_k(pyiid_ctx* c, int nq, bool weighted, int* k) {
+  std::vector<std::pair<int, int64_t>> cand;
+  CHK(sym_candidates(c, nq, weighted, cand));
+  *k = cand.empty() ? 0 : cand.front().first;
+  for (const auto& x : cand)
+    if (x.second >= 4 * 256) *k = x.first;
+  return PYIID_OK;
+}
+
 hipEvent_t take_event(pyiid_ctx* c) {
   if (!c->ev_pool.empty()) {
     hipEvent_t e = c->ev_pool.back();
@@ -643,6 +812,32 @@ int run_fused(pyiid_ctx* c, int slot, double qbin, bool tune = true) {
                            c->fpart.as<double>()));
   CHK(tm.done());
   return PYIID_OK;
+}
+
+// Symmetric kernel over the single chain: gradient slots and F partials.
+int run_sym(pyiid_ctx* c, int slot, double qbin) {
+  Table& t = c->tab[slot];
+  const int nq = t.nq;
+  CHK(ensure_packed(c));
+  CHK(ensure_norm(c, slot));
+  CHK(ensure_sym_items(c, weighted_table(t), c->sym_k));
+  const SymSet& s = c->sym;
+  CHK(c->fpart.reserve(sizeof(double) * (size_t)std::max(1, s.n_items) * nq));
+  CHK(c->gpart.reserve(sym_slot_bytes(s.n_slots, nq)));
+  const double dq = static_cast<double>(static_cast<float>(qbin));
+  Timed tm(c, 0);
+  HIPCHK(launch_sym_grad(c->stream, c->sym_m, c->pos.as<AtomRec>(), s.perm.as<int32_t>(),
+                         s.tiles.as<Range>(), s.tile_el.as<int>(), t.tab.as<float>(), nq, dq,
+                         s.items.as<SymItem>(), s.n_items, weighted_table(t), c->gpart.as<float>(),
+                         c->fpart.as<double>()));
+  CHK(tm.done());
+  return PYIID_OK;
+}
+
+hipError_t reduce_sym(pyiid_ctx* c, const Table& t, void* out, bool out_f64) {
+  return launch_reduce_slots(c->stream, c->gpart.as<float>(), c->sym.entries.as<int>(),
+                             c->sym.tiles.as<Range>(), c->sym.n_tiles, c->sym.perm.as<int32_t>(),
+                             t.nq, t.gs.as<double>(), out, out_f64);
 }
 
 // Tile pairs (I, J <= I) of every chain for the contracted-force kernel, the
@@ -886,6 +1081,84 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
   c->fused_waves = best_w;
   c->tuned[key] = {best_m, best_w};
   return PYIID_OK;
+}
+
+// Kernel of the F + gradient path: the symmetric kernel at the fastest group
+// size, or the ordered kernel (its own shapes tuned by tune_fused) when that
+// is faster, when compensated sums are on or when its bins per wave are fixed.
+int tune_grad(pyiid_ctx* c, int slot, double qbin) {
+  Table& t = c->tab[slot];
+  const bool weighted = weighted_table(t);
+  if (c->compensated || c->grad_m_fixed) {
+    c->sym_k = 0;
+    return tune_fused(c, slot, qbin);
+  }
+  if (c->sym_k_fixed) {
+    c->sym_k = c->sym_k_fixed;
+    c->sym_m = c->sym_m_fixed ? c->sym_m_fixed : 16;
+    return PYIID_OK;
+  }
+  if (c->capturing) {
+    c->sym_k = 0;
+    return tune_fused(c, slot, qbin);
+  }
+  if (!c->autotune) {
+    CHK(default_sym_k(c, t.nq, weighted, &c->sym_k));
+    c->sym_m = c->sym_m_fixed ? c->sym_m_fixed : 16;
+    return c->sym_k ? PYIID_OK : tune_fused(c, slot, qbin);
+  }
+  const std::vector<int64_t> key{5, c->n, c->n_chain, t.nq, weighted ? 1 : 0,
+                                 static_cast<int64_t>(c->struct_stamp), c->sym_m_fixed};
+  auto hit = c->tuned.find(key);
+  if (hit != c->tuned.end()) {
+    c->sym_k = static_cast<int>(hit->second[0]);
+    c->sym_m = static_cast<int>(hit->second[1]);
+    return c->sym_k ? PYIID_OK : tune_fused(c, slot, qbin);
+  }
+  std::vector<std::pair<int, int64_t>> cand;
+  CHK(sym_candidates(c, t.nq, weighted, cand));
+  CHK(c->io_b.reserve(sizeof(float) * static_cast<size_t>(c->n) * 3 * std::max(1, t.nq)));
+  double best = 1e300;
+  int best_k = 0, best_m = 16;
+  // a candidate is timed with the reduction of its partials
+  const std::vector<int> ms_shapes =
+      c->sym_m_fixed ? std::vector<int>{c->sym_m_fixed} : std::vector<int>{16, 12};
+  for (const auto& x : cand)
+    for (int m : ms_shapes) {
+      c->sym_k = x.first;
+      c->sym_m = m;
+      auto trial = [&]() {
+        CHK(run_sym(c, slot, qbin));
+        HIPCHK(reduce_sym(c, t, c->io_b.p, false));
+        return PYIID_OK;
+      };
+      double ms = 0.0;
+      CHK(trial());
+      CHK(time_launch(c, trial, &ms));
+      if (ms < best) {
+        best = ms;
+        best_k = x.first;
+        best_m = m;
+      }
+    }
+  // the ordered kernel at its default shape
+  c->sym_k = 0;
+  c->grad_m = kGradM;
+  c->fused_waves = kDefaultFusedWaves;
+  auto trial = [&]() {
+    CHK(run_fused(c, slot, qbin, false));
+    HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, c->n, t.nq,
+                              t.gs.as<double>(), c->io_b.p, false));
+    return PYIID_OK;
+  };
+  double ms = 0.0;
+  CHK(trial());
+  CHK(time_launch(c, trial, &ms));
+  if (ms < best) best_k = 0;
+  c->sym_k = best_k;
+  c->sym_m = best_m;
+  c->tuned[key] = {best_k, best_m};
+  return best_k ? PYIID_OK : tune_fused(c, slot, qbin);
 }
 
 // forces (n x 3 doubles) = scale * sum_q u[chain][q] grad(i, c, q), by the
@@ -1533,6 +1806,22 @@ int pyiid_set_grad_bins_per_wave(pyiid_ctx* c, int m) {
   return PYIID_OK;
 }
 
+int pyiid_set_grad_tiles_per_item(pyiid_ctx* c, int k) {
+  CHK(check_ctx(c));
+  if (k != 0 && k != 4 && k != 8 && k != 16 && k != 32)
+    return fail(PYIID_E_INVALID, "tiles per item %d is not 0, 4, 8, 16 or 32", k);
+  c->sym_k_fixed = k;
+  return PYIID_OK;
+}
+
+int pyiid_set_grad_sym_bins_per_wave(pyiid_ctx* c, int m) {
+  CHK(check_ctx(c));
+  if (m != 0 && m != 12 && m != 16)
+    return fail(PYIID_E_INVALID, "symmetric bins per wave %d is not 0, 12 or 16", m);
+  c->sym_m_fixed = m;
+  return PYIID_OK;
+}
+
 int pyiid_kernel_time(pyiid_ctx* c, int kind, double* total_ms, int64_t* launches) {
   CHK(check_ctx(c));
   if (kind < 0 || kind > 4) return fail(PYIID_E_INVALID, "kernel kind %d not in [0, 4]", kind);
@@ -1671,23 +1960,35 @@ int pyiid_grad_fq(pyiid_ctx* c, int slot, double qbin, void* grad, float* fq, in
     }
     return finish(c, flags);
   }
-  CHK(run_fused(c, slot, qbin));
+  CHK(ensure_packed(c));
+  CHK(ensure_norm(c, slot));
+  CHK(tune_grad(c, slot, qbin));
+  const bool sym = c->sym_k != 0;
+  if (sym)
+    CHK(run_sym(c, slot, qbin));
+  else
+    CHK(run_fused(c, slot, qbin, false));
   Table& t = c->tab[slot];
   void* gdst = grad;
   if (!(flags & PYIID_DEVICE)) {
     CHK(c->io_c.reserve(gbytes));
     gdst = c->io_c.p;
   }
-  HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, n, nq,
-                            t.gs.as<double>(), gdst, (flags & PYIID_F64) != 0));
+  if (sym)
+    HIPCHK(reduce_sym(c, t, gdst, (flags & PYIID_F64) != 0));
+  else
+    HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, n, nq,
+                              t.gs.as<double>(), gdst, (flags & PYIID_F64) != 0));
   if (!(flags & PYIID_DEVICE)) CHK(download(c, grad, gdst, gbytes, flags));
   if (fq) {
-    // F from all items of the single chain (the range stored after the tile ranges)
+    // F from all items of the single chain (for the ordered kernel, the range
+    // stored after the tile ranges)
     CHK(c->io_a.reserve(sizeof(float) * nq));
     float* fdst = (flags & PYIID_DEVICE) ? fq : c->io_a.as<float>();
-    HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(),
-                                c->fused_items.ranges.as<Range>() + c->fused_items.n_ranges, 1,
-                                nq, t.fsu.as<double>(), nullptr, fdst));
+    const Range* fr = sym ? c->sym.franges.as<Range>()
+                          : c->fused_items.ranges.as<Range>() + c->fused_items.n_ranges;
+    HIPCHK(launch_reduce_ranges(c->stream, c->fpart.as<double>(), fr, 1, nq, t.fsu.as<double>(),
+                                nullptr, fdst));
     if (!(flags & PYIID_DEVICE)) CHK(download(c, fq, fdst, sizeof(float) * nq, flags));
   }
   return finish(c, flags);

@@ -45,6 +45,27 @@ hipError_t launch_fused_grad(hipStream_t st, int m, bool compensated, const Atom
                              const int32_t* jperm, int n_total, const float* ftab, int nq,
                              double dq, const Item* items, int n_items, bool weighted,
                              float* gpart, double* fpart);
+// Work item of the symmetric F + gradient kernel: 64-position tiles [g0, g1)
+// against tiles [r0, r1) of one element, pairs (I, J) with J <= I.  The I
+// rows of tile I go to slot islot0 + (I - max(g0, r0)), the J rows of tile J
+// to slot jslot0 + (J - r0).
+struct SymItem {
+  int g0, g1, r0, r1, islot0, jslot0, pad0, pad1;
+};
+// Unordered-pair F + gradient: slots[slot][c][q][64] (float; tiles[t] =
+// {slot list begin, end, position begin, end}, positions in perm) and
+// fpart[item][q] (F partial, double).  Weighted: tile_el[t] is the element
+// of tile t.
+// m = 16 (8 waves, 128 bins per block) or 12 (12 waves, 144 bins).
+hipError_t launch_sym_grad(hipStream_t st, int m, const AtomRec* pos, const int32_t* perm,
+                           const Range* tiles, const int* tile_el, const float* ftab, int nq,
+                           double dq, const SymItem* items, int n_items, bool weighted,
+                           float* slots, double* fpart);
+// out[perm[p]][c][q] = gscale[q] * sum_{e in tile} slots[entries[e]][c][q][row]
+// (FP64, list order), float or double output.
+hipError_t launch_reduce_slots(hipStream_t st, const float* slots, const int* entries,
+                               const Range* tiles, int n_tiles, const int32_t* perm, int nq,
+                               const double* gscale, void* out, bool out_f64);
 // Contracted forces over unordered pairs: items are tile pairs (I, J), J <= I,
 // of at most 64 positions of one chain in perm (Item.tile = I's and Item.pad =
 // J's chain-global tile index; weighted: Item.split = e_I | e_J << 16, each
