@@ -105,7 +105,8 @@ int pyiid_set_grad_sym_bins_per_wave(pyiid_ctx *ctx, int m);
 /* Autotuning of the Debye kernels' launch shapes (F + gradient kernel and its
  * grouping, F-only bins per wave, contracted-force Q groups), on by
  * default: the first call for a problem shape times the candidates and keeps
- * the fastest.  Off (enable = 0): every call uses fixed default shapes, so
+ * the fastest.  Off (enable = 0): every call uses fixed default shapes (the
+ * ordered F + gradient kernel unless pyiid_set_grad_tiles_per_item is set), so
  * results are the same bit for bit from run to run and context to context;
  * with it on they agree to FP32 rounding.  No counterpart in the reference. */
 int pyiid_set_autotune(pyiid_ctx *ctx, int enable);

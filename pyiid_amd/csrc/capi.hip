@@ -674,29 +674,15 @@ int ensure_sym_items(pyiid_ctx* c, bool weighted, int k) {
   return PYIID_OK;
 }
 
-// Group sizes of the symmetric kernel whose slots fit kSymSlotBudget, with
-// their block counts at 128 bins per block.
-int sym_candidates(pyiid_ctx* c, int nq, bool weighted, std::vector<std::pair<int, int64_t>>& out) {
+// Group sizes of the symmetric kernel whose slots fit kSymSlotBudget.
+int sym_candidates(pyiid_ctx* c, int nq, bool weighted, std::vector<int>& out) {
   out.clear();
   SymPlan pl;
   CHK(plan_sym_tiles(c, weighted, pl));
-  const int64_t n_qg = (nq + 127) / 128;
   for (int k : kSymGroups) {
     CHK(plan_sym_items(pl, k));
-    if (sym_slot_bytes(pl.n_slots, nq) > kSymSlotBudget) continue;
-    out.emplace_back(k, static_cast<int64_t>(pl.items.size()) * n_qg);
+    if (sym_slot_bytes(pl.n_slots, nq) <= kSymSlotBudget) out.push_back(k);
   }
-  return PYIID_OK;
-}
-
-// Group size without the autotuner: the largest with at least 4 blocks per
-// CU (256 CUs), else the smallest that fits; 0 (ordered kernel) if none fits.
-int default_sym_k(pyiid_ctx* c, int nq, bool weighted, int* k) {
-  std::vector<std::pair<int, int64_t>> cand;
-  CHK(sym_candidates(c, nq, weighted, cand));
-  *k = cand.empty() ? 0 : cand.front().first;
-  for (const auto& x : cand)
-    if (x.second >= 4 * 256) *k = x.first;
   return PYIID_OK;
 }
 
@@ -1084,8 +1070,9 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
 }
 
 // Kernel of the F + gradient path: the symmetric kernel at the fastest group
-// size, or the ordered kernel (its own shapes tuned by tune_fused) when that
-// is faster, when compensated sums are on or when its bins per wave are fixed.
+// size and shape, or the ordered kernel (its own shapes tuned by tune_fused)
+// when that is faster, when compensated sums are on, when its bins per wave
+// are fixed or when the autotuner is off.
 int tune_grad(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
   const bool weighted = weighted_table(t);
@@ -1103,9 +1090,8 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
     return tune_fused(c, slot, qbin);
   }
   if (!c->autotune) {
-    CHK(default_sym_k(c, t.nq, weighted, &c->sym_k));
-    c->sym_m = c->sym_m_fixed ? c->sym_m_fixed : 16;
-    return c->sym_k ? PYIID_OK : tune_fused(c, slot, qbin);
+    c->sym_k = 0;
+    return tune_fused(c, slot, qbin);
   }
   const std::vector<int64_t> key{5, c->n, c->n_chain, t.nq, weighted ? 1 : 0,
                                  static_cast<int64_t>(c->struct_stamp), c->sym_m_fixed};
@@ -1115,7 +1101,7 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
     c->sym_m = static_cast<int>(hit->second[1]);
     return c->sym_k ? PYIID_OK : tune_fused(c, slot, qbin);
   }
-  std::vector<std::pair<int, int64_t>> cand;
+  std::vector<int> cand;
   CHK(sym_candidates(c, t.nq, weighted, cand));
   CHK(c->io_b.reserve(sizeof(float) * static_cast<size_t>(c->n) * 3 * std::max(1, t.nq)));
   double best = 1e300;
@@ -1123,9 +1109,9 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
   // a candidate is timed with the reduction of its partials
   const std::vector<int> ms_shapes =
       c->sym_m_fixed ? std::vector<int>{c->sym_m_fixed} : std::vector<int>{16, 12};
-  for (const auto& x : cand)
+  for (int k : cand)
     for (int m : ms_shapes) {
-      c->sym_k = x.first;
+      c->sym_k = k;
       c->sym_m = m;
       auto trial = [&]() {
         CHK(run_sym(c, slot, qbin));
@@ -1137,7 +1123,7 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
       CHK(time_launch(c, trial, &ms));
       if (ms < best) {
         best = ms;
-        best_k = x.first;
+        best_k = k;
         best_m = m;
       }
     }

@@ -47,16 +47,15 @@ constexpr int kSymS = 16;  // steps per phase (one setup round)
 
 // Launch shapes: M Q bins per wave, W waves per block (M W bins per block):
 // 16 x 8 (2 waves per SIMD) and 12 x 12 (3 per SIMD, within 168 VGPRs).
-// LDS: per (step, lane) float4 {1/r, (q_i - q_j)/r^2} and float4 {sin
-// dtheta, hi, 4 lo, 0}, per (wave, step, lane) float2 {sin, cos(Q0 r)}; then
-// the J tile's atoms.  The FP64 lane reduction reuses the start.
+// LDS: the setup as FP32 fields [field][step][lane] (1/r, (q_i - q_j)/r^2,
+// sin dtheta, hi, 4 lo, then sin, cos(Q0 r) of each wave's chunk), then the
+// J tile's atoms.  The FP64 lane reduction reuses the start.
 template <int M, int W>
 struct SymShape {
   static constexpr int kBins = M * W;
-  static constexpr int kA = 0;
-  static constexpr int kB = kA + kSymS * 64 * 16;
-  static constexpr int kAnc = kB + kSymS * 64 * 16;
-  static constexpr int kTile = kAnc + W * kSymS * 64 * 8;
+  static constexpr int kFields = 7 + 2 * W;
+  static constexpr int kGS = kSymS * 64;  // field stride (floats)
+  static constexpr int kTile = kFields * kGS * 4;
   static constexpr int kEnd = kTile + 64 * 32;
   static constexpr int kRed = W * kRedM * kRedRow * 4;
   static constexpr int kSmem = kEnd > kRed ? kEnd : kRed;
@@ -120,9 +119,8 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
   using Sh = SymShape<M, W>;
   constexpr int kSymBins = Sh::kBins;
   __shared__ __attribute__((aligned(32))) unsigned char smem[Sh::kSmem];
-  float4* geo_a = reinterpret_cast<float4*>(smem + Sh::kA);
-  float4* geo_b = reinterpret_cast<float4*>(smem + Sh::kB);
-  float2* geo_anc = reinterpret_cast<float2*>(smem + Sh::kAnc);
+  constexpr int GS = Sh::kGS;
+  float* geo = reinterpret_cast<float*>(smem);
   AtomRec* jt = reinterpret_cast<AtomRec*>(smem + Sh::kTile);
 
   const int item = blockIdx.x / n_qg;
@@ -195,8 +193,13 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
           float s1, k2hi, k2lo;
           two_cos(dth, k2hi, k2lo, s1);
           const int o = k * 64 + lane;
-          geo_a[o] = make_float4(g.inv_r, g.dpx, g.dpy, g.dpz);
-          geo_b[o] = make_float4(s1, k2hi, 4.0f * k2lo, 0.0f);
+          geo[o] = g.inv_r;
+          geo[o + GS] = g.dpx;
+          geo[o + 2 * GS] = g.dpy;
+          geo[o + 3 * GS] = g.dpz;
+          geo[o + 4 * GS] = s1;
+          geo[o + 5 * GS] = k2hi;
+          geo[o + 6 * GS] = 4.0f * k2lo;
           // anchors sin, cos(Q0 r) of the W waves' chunks: the first reduced
           // in FP64, the others by FP32 rotations by M dtheta
           float as, ac, rsn, rcs;
@@ -204,7 +207,8 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
           sincos_red(static_cast<double>(M) * dth, rsn, rcs);
 #pragma unroll
           for (int w = 0; w < W; ++w) {
-            geo_anc[(w * kSymS + k) * 64 + lane] = make_float2(as, ac);
+            geo[o + (7 + 2 * w) * GS] = as;
+            geo[o + (8 + 2 * w) * GS] = ac;
             const float as2 = fmaf(as, rcs, ac * rsn);
             ac = fmaf(ac, rcs, -as * rsn);
             as = as2;
@@ -215,28 +219,28 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
         // written): a branch around them would split the accumulators'
         // registers between two paths.
         //
-        // g: own {1/r, dp}, own {sin dtheta, hi, 4 lo}, own anchor, partner dp
+        // g: 1/r, dpx, dpy, dpz, sin dtheta, hi, 4 lo, anchor sin, cos
         struct G {
-          float4 a, b, p;
-          float2 anc;
+          float f[9];
         };
         auto load = [&](G& g, int k) {
           const int o = k * 64 + lane;
-          g.a = geo_a[o];
-          g.b = geo_b[o];
-          g.anc = geo_anc[(wave * kSymS + k) * 64 + lane];
-          g.p = geo_a[k * 64 + 16 * row + ((place - k) & 15)];
+#pragma unroll
+          for (int f = 0; f < 7; ++f) g.f[f] = geo[o + f * GS];
+          g.f[7] = geo[o + (7 + 2 * wave) * GS];
+          g.f[8] = geo[o + (8 + 2 * wave) * GS];
         };
         // One step: per bin, t from the recurrences; the I sums take t dp,
-        // the J sums the partner's t (row_ror:S) times the partner's dp (the
-        // sign is applied when the slot is written).
+        // the J sums the partner's t (row_ror:S) times the partner's dp (its
+        // lane's dp through the same rotation; the sign is applied when the
+        // slot is written).
         auto step = [&](auto sc, const G& g) {
           constexpr int S = decltype(sc)::value;
-          const float inv_r = g.a.x, dpx = g.a.y, dpy = g.a.z, dpz = g.a.w;
-          const float s1 = g.b.x, k2 = g.b.y, lo4 = g.b.z;
-          const float px = g.p.y, py = g.p.z, pz = g.p.w;
-          const float sn = g.anc.x;
-          float cs = g.anc.y;
+          const float inv_r = g.f[0], dpx = g.f[1], dpy = g.f[2], dpz = g.f[3];
+          const float s1 = g.f[4], k2 = g.f[5], lo4 = g.f[6];
+          const float px = row_ror<S>(dpx), py = row_ror<S>(dpy), pz = row_ror<S>(dpz);
+          const float sn = g.f[7];
+          float cs = g.f[8];
           const float c1 = 0.5f * k2;
           float w = sn * inv_r;
           float wp = fmaf(sn, c1, -cs * s1) * inv_r;  // sin(Q0 r - dtheta) / r
