@@ -36,11 +36,12 @@ def workload():
     return pos, table, index
 
 
-def read_traffic(label="au6699_q1250"):
-    """HBM bytes per launch of the fused kernel on workload `label` from the
-    committed rocprof counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the
-    copy-kernel calibration), or None when it has not been measured."""
-    path = os.path.join(ROOT, "profiles", "fused_grad_%s_traffic.txt" % label)
+def read_traffic(label="au6699_q1250", kernel="fused_grad"):
+    """HBM bytes per launch of the F + gradient kernel (`kernel`: fused_grad,
+    the ordered one, or sym_grad) on workload `label` from the committed
+    rocprof counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when it has not been measured."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (kernel, label))
     if not os.path.exists(path):
         return None
     vals = {}
@@ -53,6 +54,26 @@ def read_traffic(label="au6699_q1250"):
         return float(vals["traffic_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
+
+
+GRAD_KERNEL = "auto"  # --grad-kernel
+
+
+def grad_kernel_setup(ctx):
+    """Restrict the F + gradient path to the kernel chosen by --grad-kernel."""
+    from pyiid_amd import _lib
+    ctx.set_grad_kernel({"auto": _lib.GRAD_AUTO, "ordered": _lib.GRAD_ORDERED,
+                         "symmetric": _lib.GRAD_SYMMETRIC}[GRAD_KERNEL])
+
+
+def grad_kernel_name(ctx):
+    """(profile name, description) of the kernel the last grad_fq ran."""
+    sym, bpw, group = ctx.grad_shape()
+    if sym:
+        return "sym_grad", ("sym_grad_kernel (F + grad, unordered pairs), %d bins per wave, "
+                            "%d tiles per item" % (bpw, group))
+    return "fused_grad", ("fused_grad_kernel (F + grad, ordered pairs), %d bins per wave, "
+                          "%d j ranges per tile" % (bpw, group))
 
 
 def host_cpu():
@@ -290,6 +311,7 @@ def headline_20k(steps=3, warmup=1):
         ctx.set_positions(pos)
         ctx.set_elements(index)
         ctx.set_scatter(_lib.SLOT_FQ, table)
+        grad_kernel_setup(ctx)
         g = torch.empty((n, 3, nq), dtype=torch.float32, device="cuda")
         f = torch.empty(nq, dtype=torch.float32, device="cuda")
 
@@ -308,14 +330,15 @@ def headline_20k(steps=3, warmup=1):
         wall = (time.perf_counter() - t0) / steps
         ms, k = ctx.kernel_time(_lib.KERNEL_FUSED)
         ctx.enable_timing(False)
+        kname, kdesc = grad_kernel_name(ctx)
         ctx.close()
         pairs = n * (n - 1) // 2
         avg_s = ms / max(1, k) / 1e3
         achieved = FLOP_PER_PAIR_Q * pairs * nq / avg_s / 1e12
         out[label] = {"atoms": n, "q_bins": nq, "steps": steps, "ms_per_step": wall * 1e3,
-                      "pair_q_per_s": pairs * nq / wall,
+                      "pair_q_per_s": pairs * nq / wall, "kernel": kdesc,
                       "fused_avg_launch_ms": avg_s * 1e3, "achieved_tflops": achieved,
-                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(label)}
+                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(label, kname)}
         del g, f
     return out
 
@@ -468,6 +491,7 @@ def kernel_suite(reps=3):
     c20.set_positions(pos20)
     c20.set_elements(i20)
     c20.set_scatter(_lib.SLOT_FQ, t20)
+    grad_kernel_setup(c20)
     n20 = len(pos20)
     p20 = n20 * (n20 - 1) // 2
     g20 = torch.empty((n20, 3, NQ), dtype=torch.float32, device="cuda")
@@ -478,7 +502,7 @@ def kernel_suite(reps=3):
     ms = timed(c20, _lib.KERNEL_FUSED, lambda: _lib.check(c20.lib.pyiid_grad_fq(
         c20.handle, _lib.SLOT_FQ, ctypes.c_double(QBIN), ctypes.c_void_p(g20.data_ptr()),
         ctypes.c_void_p(f20.data_ptr()), dev)))
-    out["fused_grad_kernel_20k"] = {"ms": ms, "atoms": n20,
+    out["fused_grad_kernel_20k"] = {"ms": ms, "atoms": n20, "kernel": grad_kernel_name(c20)[1],
                                     "pair_q_per_s": p20 * NQ / (ms * 1e-3),
                                     "tflops_alg": FLOP_PER_PAIR_Q * p20 * NQ / (ms * 1e-3) / 1e12}
     del g20, f20
@@ -491,6 +515,7 @@ def kernel_suite(reps=3):
     ca.set_positions(posa)
     ca.set_elements(ia)
     ca.set_scatter(_lib.SLOT_FQ, ta)
+    grad_kernel_setup(ca)
     pa = 20000 * 19999 // 2
     ga = torch.empty((20000, 3, 1500), dtype=torch.float32, device="cuda")
     fa = torch.empty(1500, dtype=torch.float32, device="cuda")
@@ -500,7 +525,8 @@ def kernel_suite(reps=3):
     ms = timed(ca, _lib.KERNEL_FUSED, lambda: _lib.check(ca.lib.pyiid_grad_fq(
         ca.handle, _lib.SLOT_FQ, ctypes.c_double(QBIN), ctypes.c_void_p(ga.data_ptr()),
         ctypes.c_void_p(fa.data_ptr()), dev)))
-    out["fused_grad_kernel_alloy20k"] = {"ms": ms, "pair_q_per_s": pa * 1500 / (ms * 1e-3),
+    out["fused_grad_kernel_alloy20k"] = {"ms": ms, "kernel": grad_kernel_name(ca)[1],
+                                         "pair_q_per_s": pa * 1500 / (ms * 1e-3),
                                          "tflops_alg": FLOP_PER_PAIR_Q * pa * 1500 / (ms * 1e-3) / 1e12}
     del ga, fa
     ca.close()
@@ -583,10 +609,15 @@ def main():
                     help="only run the exact / learned-integrator sampler comparison")
     ap.add_argument("--hmc-proposals", type=int, default=40,
                     help="exact-HMC proposals per seed of the sampler comparison (0 skips it)")
+    ap.add_argument("--grad-kernel", choices=("auto", "ordered", "symmetric"), default="auto",
+                    help="F + gradient kernel of the timed steps and the 20k workloads "
+                         "(default: the autotuner's choice)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times (the known-size "
                          "reference for the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    global GRAD_KERNEL
+    GRAD_KERNEL = args.grad_kernel
     if args.copy_calibration:
         import torch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -623,6 +654,7 @@ def main():
     ctx.set_positions(pos)
     ctx.set_elements(index)
     ctx.set_scatter(_lib.SLOT_FQ, table)
+    grad_kernel_setup(ctx)
     grad = torch.empty((n, 3, NQ), dtype=torch.float32, device="cuda")
     fq = torch.empty(NQ, dtype=torch.float32, device="cuda")
     flags = _lib.PYIID_DEVICE
@@ -645,6 +677,7 @@ def main():
     t1 = time.perf_counter()
     fused_ms, fused_n = ctx.kernel_time(_lib.KERNEL_FUSED)
     ctx.enable_timing(False)
+    kname, kdesc = grad_kernel_name(ctx)
     elapsed = t1 - t0
     value = pairs * NQ * args.steps / elapsed
     avg_s = fused_ms / max(1, fused_n) / 1e3
@@ -673,13 +706,13 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "fused_grad_kernel (F + grad, ordered pairs)",
+            "kernel": kdesc,
             "bound": "valu",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": read_traffic(),
+            "traffic": read_traffic("au6699_q1250", kname),
             "avg_launch_ms": avg_s * 1e3,
             "launches": fused_n,
             "flop_per_unit": FLOP_PER_PAIR_Q,

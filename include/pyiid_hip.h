@@ -64,7 +64,8 @@ int pyiid_set_stream(pyiid_ctx *ctx, void *stream);
 int pyiid_synchronize(pyiid_ctx *ctx);
 /* Kernel timing with HIP events on the launch stream: when enabled, every
  * launch of the Debye kernels is bracketed by events and its duration summed.
- * kind: 0 = fused F+grad kernel, 1 = F-only kernel, 2 = transform GEMM,
+ * kind: 0 = F+grad kernel (ordered or symmetric, pyiid_grad_shape tells
+ *       which), 1 = F-only kernel, 2 = transform GEMM,
  *       3 = pair-geometry kernel, 4 = contracted-force (unordered pair) kernel. */
 int pyiid_enable_timing(pyiid_ctx *ctx, int enable);
 int pyiid_kernel_time(pyiid_ctx *ctx, int kind, double *total_ms, int64_t *launches);
@@ -102,6 +103,20 @@ int pyiid_set_grad_tiles_per_item(pyiid_ctx *ctx, int k);
  * lets the autotuner time both.  The results agree to FP32 rounding.  No
  * counterpart in the reference. */
 int pyiid_set_grad_sym_bins_per_wave(pyiid_ctx *ctx, int m);
+/* Kernel of pyiid_grad_fq: PYIID_GRAD_AUTO (the default) lets the autotuner
+ * choose, PYIID_GRAD_ORDERED and PYIID_GRAD_SYMMETRIC restrict it to one
+ * kernel (whose launch shapes are still autotuned unless fixed by the setters
+ * above).  Compensated sums always run the ordered kernel.  No counterpart in
+ * the reference. */
+#define PYIID_GRAD_AUTO 0
+#define PYIID_GRAD_ORDERED 1
+#define PYIID_GRAD_SYMMETRIC 2
+int pyiid_set_grad_kernel(pyiid_ctx *ctx, int kernel);
+/* Launch shape of the last pyiid_grad_fq call: *symmetric = 1 for the
+ * symmetric (unordered-pair) kernel, with its Q bins per wave and tiles per
+ * item in *group; 0 for the ordered kernel, with its Q bins per wave and its
+ * number of j ranges per tile in *group.  No counterpart in the reference. */
+int pyiid_grad_shape(pyiid_ctx *ctx, int *symmetric, int *bins_per_wave, int *group);
 /* Autotuning of the Debye kernels' launch shapes (F + gradient kernel and its
  * grouping, F-only bins per wave, contracted-force Q groups), on by
  * default: the first call for a problem shape times the candidates and keeps

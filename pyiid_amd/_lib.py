@@ -20,6 +20,9 @@ POT_RW = 0
 POT_CHI_SQ = 1
 PRECISION_DEFAULT = 0
 PRECISION_COMPENSATED = 1
+GRAD_AUTO = 0
+GRAD_ORDERED = 1
+GRAD_SYMMETRIC = 2
 
 KERNEL_FUSED = 0
 KERNEL_FQ = 1
@@ -55,6 +58,9 @@ _SIGNATURES = [
     ("pyiid_set_grad_bins_per_wave", [_P, _I]),
     ("pyiid_set_grad_tiles_per_item", [_P, _I]),
     ("pyiid_set_grad_sym_bins_per_wave", [_P, _I]),
+    ("pyiid_set_grad_kernel", [_P, _I]),
+    ("pyiid_grad_shape", [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                          ctypes.POINTER(ctypes.c_int)]),
     ("pyiid_set_autotune", [_P, _I]),
     ("pyiid_set_precision", [_P, _I]),
     ("pyiid_set_positions", [_P, _P, _I64, _I]),
@@ -210,6 +216,18 @@ class Context:
     def set_grad_sym_bins_per_wave(self, m):
         """Fix the symmetric F + gradient kernel's Q bins per wave (12, 16), or 0 to autotune."""
         check(self.lib.pyiid_set_grad_sym_bins_per_wave(self.handle, int(m)))
+
+    def set_grad_kernel(self, kernel):
+        """GRAD_AUTO, GRAD_ORDERED or GRAD_SYMMETRIC: the kernel of grad_fq."""
+        check(self.lib.pyiid_set_grad_kernel(self.handle, int(kernel)))
+
+    def grad_shape(self):
+        """Launch shape of the last grad_fq: (symmetric, bins per wave, group), where group is
+        the symmetric kernel's tiles per item or the ordered kernel's j ranges per tile."""
+        a, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.pyiid_grad_shape(self.handle, ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(g)))
+        return bool(a.value), b.value, g.value
 
     def kernel_time(self, kind):
         ms = ctypes.c_double()

@@ -201,6 +201,7 @@ struct pyiid_ctx {
   int sym_k_fixed = 0;  // set by pyiid_set_grad_tiles_per_item (0: autotuned)
   int sym_m = 16;       // Q bins per wave of the symmetric kernel (16 or 12)
   int sym_m_fixed = 0;  // set by pyiid_set_grad_sym_bins_per_wave (0: autotuned)
+  int grad_kernel = PYIID_GRAD_AUTO;  // set by pyiid_set_grad_kernel
   int64_t pf_waves = kDefaultPfWaves;        // target wave count that sets the force Q groups
   bool autotune = true;
   bool compensated = false;  // compensated FP32 sums in the F and F + gradient kernels
@@ -1076,7 +1077,7 @@ int tune_fused(pyiid_ctx* c, int slot, double qbin) {
 int tune_grad(pyiid_ctx* c, int slot, double qbin) {
   Table& t = c->tab[slot];
   const bool weighted = weighted_table(t);
-  if (c->compensated || c->grad_m_fixed) {
+  if (c->compensated || c->grad_m_fixed || c->grad_kernel == PYIID_GRAD_ORDERED) {
     c->sym_k = 0;
     return tune_fused(c, slot, qbin);
   }
@@ -1085,16 +1086,20 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
     c->sym_m = c->sym_m_fixed ? c->sym_m_fixed : 16;
     return PYIID_OK;
   }
-  if (c->capturing) {
-    c->sym_k = 0;
-    return tune_fused(c, slot, qbin);
-  }
-  if (!c->autotune) {
-    c->sym_k = 0;
-    return tune_fused(c, slot, qbin);
+  const bool sym_only = c->grad_kernel == PYIID_GRAD_SYMMETRIC;
+  // without the autotuner, or first met under graph capture: default shapes
+  if (!c->autotune || c->capturing) {
+    if (!sym_only) {
+      c->sym_k = 0;
+      return tune_fused(c, slot, qbin);
+    }
+    c->sym_k = 16;
+    c->sym_m = c->sym_m_fixed ? c->sym_m_fixed : 16;
+    return PYIID_OK;
   }
   const std::vector<int64_t> key{5, c->n, c->n_chain, t.nq, weighted ? 1 : 0,
-                                 static_cast<int64_t>(c->struct_stamp), c->sym_m_fixed};
+                                 static_cast<int64_t>(c->struct_stamp), c->sym_m_fixed,
+                                 c->grad_kernel};
   auto hit = c->tuned.find(key);
   if (hit != c->tuned.end()) {
     c->sym_k = static_cast<int>(hit->second[0]);
@@ -1103,10 +1108,31 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
   }
   std::vector<int> cand;
   CHK(sym_candidates(c, t.nq, weighted, cand));
+  if (sym_only && cand.empty()) cand.push_back(kSymGroups[3]);
   CHK(c->io_b.reserve(sizeof(float) * static_cast<size_t>(c->n) * 3 * std::max(1, t.nq)));
+  // every candidate is timed twice (the faster counts) with the reduction of
+  // its partials; the ordered kernel at the shape its own tuner chose
+  auto time2 = [&](auto trial, double* best_ms) {
+    double a = 0.0, b = 0.0;
+    CHK(trial());
+    CHK(time_launch(c, trial, &a));
+    CHK(time_launch(c, trial, &b));
+    *best_ms = std::min(a, b);
+    return PYIID_OK;
+  };
   double best = 1e300;
   int best_k = 0, best_m = 16;
-  // a candidate is timed with the reduction of its partials
+  if (!sym_only) {
+    c->sym_k = 0;
+    CHK(tune_fused(c, slot, qbin));
+    auto trial = [&]() {
+      CHK(run_fused(c, slot, qbin, false));
+      HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, c->n,
+                                t.nq, t.gs.as<double>(), c->io_b.p, false));
+      return PYIID_OK;
+    };
+    CHK(time2(trial, &best));
+  }
   const std::vector<int> ms_shapes =
       c->sym_m_fixed ? std::vector<int>{c->sym_m_fixed} : std::vector<int>{16, 12};
   for (int k : cand)
@@ -1119,31 +1145,17 @@ int tune_grad(pyiid_ctx* c, int slot, double qbin) {
         return PYIID_OK;
       };
       double ms = 0.0;
-      CHK(trial());
-      CHK(time_launch(c, trial, &ms));
+      CHK(time2(trial, &ms));
       if (ms < best) {
         best = ms;
         best_k = k;
         best_m = m;
       }
     }
-  // the ordered kernel at its default shape
-  c->sym_k = 0;
-  c->grad_m = kGradM;
-  c->fused_waves = kDefaultFusedWaves;
-  auto trial = [&]() {
-    CHK(run_fused(c, slot, qbin, false));
-    HIPCHK(launch_reduce_grad(c->stream, c->gpart.as<float>(), c->fused_items.split, c->n, t.nq,
-                              t.gs.as<double>(), c->io_b.p, false));
-    return PYIID_OK;
-  };
-  double ms = 0.0;
-  CHK(trial());
-  CHK(time_launch(c, trial, &ms));
-  if (ms < best) best_k = 0;
   c->sym_k = best_k;
   c->sym_m = best_m;
   c->tuned[key] = {best_k, best_m};
+  // (the ordered kernel's shape is cached by tune_fused)
   return best_k ? PYIID_OK : tune_fused(c, slot, qbin);
 }
 
@@ -1797,6 +1809,23 @@ int pyiid_set_grad_tiles_per_item(pyiid_ctx* c, int k) {
   if (k != 0 && k != 4 && k != 8 && k != 16 && k != 32)
     return fail(PYIID_E_INVALID, "tiles per item %d is not 0, 4, 8, 16 or 32", k);
   c->sym_k_fixed = k;
+  return PYIID_OK;
+}
+
+int pyiid_set_grad_kernel(pyiid_ctx* c, int kernel) {
+  CHK(check_ctx(c));
+  if (kernel != PYIID_GRAD_AUTO && kernel != PYIID_GRAD_ORDERED && kernel != PYIID_GRAD_SYMMETRIC)
+    return fail(PYIID_E_INVALID, "F + gradient kernel %d is not 0, 1 or 2", kernel);
+  c->grad_kernel = kernel;
+  return PYIID_OK;
+}
+
+int pyiid_grad_shape(pyiid_ctx* c, int* symmetric, int* bins_per_wave, int* group) {
+  CHK(check_ctx(c));
+  if (!symmetric || !bins_per_wave || !group) return fail(PYIID_E_INVALID, "null output");
+  *symmetric = c->sym_k ? 1 : 0;
+  *bins_per_wave = c->sym_k ? c->sym_m : c->grad_m;
+  *group = c->sym_k ? c->sym_k : c->fused_items.split;
   return PYIID_OK;
 }
 

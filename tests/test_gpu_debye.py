@@ -250,6 +250,7 @@ def test_fused_every_bins_per_wave(ctx):
                 per_atom = load(ctx, pos, numbers, qbin, nq)
                 fq_out = np.empty(nq, np.float32)
                 got = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_out)
+                assert ctx.grad_shape()[:2] == (False, m)
                 eg = rel(got, oracle.grad_fq(pos, per_atom, qbin))
                 ef = rel(fq_out, oracle.fq(pos, per_atom, qbin))
                 print("m=%d n=%d: grad %.2e F %.2e" % (m, n, eg, ef))
@@ -291,6 +292,7 @@ def test_symmetric_every_group_size(ctx):
                 per_atom = load(ctx, pos, numbers, qbin, nq)
                 fq_out = np.empty(nq, np.float32)
                 got = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_out, f64=True)
+                assert ctx.grad_shape() == (True, 16, k)
                 rows = np.arange(0, n, 7) if n > 500 else np.arange(n)
                 want = oracle.grad_fq(pos, per_atom, qbin, atom_index=rows)
                 eg = rel(got[rows], want)
@@ -303,6 +305,30 @@ def test_symmetric_every_group_size(ctx):
             ctx.set_grad_tiles_per_item(12)
     finally:
         ctx.set_grad_tiles_per_item(0)
+
+
+def test_grad_kernel_choice(ctx):
+    """set_grad_kernel restricts the autotuned F + gradient path to one kernel,
+    which grad_shape reports; both match the oracle and each other to FP32
+    rounding (a Pt/Ni alloy of 700 atoms at 400 bins)."""
+    n, nq, qbin = 700, 400, 0.06
+    pos = structures.jitter(structures.fcc_nearest(n, a=3.724), 0.05, seed=41).astype(np.float32)
+    numbers = structures.random_alloy(n, 78, 28, seed=41)
+    per_atom = load(ctx, pos, numbers, qbin, nq)
+    rows = np.arange(0, n, 9)
+    want = oracle.grad_fq(pos, per_atom, qbin, atom_index=rows)
+    got = {}
+    try:
+        for kernel, sym in ((_lib.GRAD_ORDERED, False), (_lib.GRAD_SYMMETRIC, True)):
+            ctx.set_grad_kernel(kernel)
+            got[sym] = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, f64=True)
+            assert ctx.grad_shape()[0] is sym
+            assert rel(got[sym][rows], want) < TOL
+        assert rel(got[True], got[False]) < 2e-6
+        with pytest.raises(ValueError, match="kernel"):
+            ctx.set_grad_kernel(3)
+    finally:
+        ctx.set_grad_kernel(_lib.GRAD_AUTO)
 
 
 def test_fq_bins_per_wave_rejects_other_values(ctx):
