@@ -186,11 +186,8 @@ def c1_compare(lib, threads, n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100
     proposal, ASE masses, energy 100 Rw, beta 50), seconds per proposal."""
     from oracle import native
     from oracle import scatter as S
-    from pyiid_amd import Atoms, structures
-    from pyiid_amd.calc import wrap_rw
-    from pyiid_amd.elasticscatter import ElasticScatter
+    from pyiid_amd import structures
     from pyiid_amd.formfactors import MASSES, element_table
-    from pyiid_amd.sim import DeviceHMC
     ideal = structures.cuboctahedron(4.08)
     pos = structures.jitter(ideal, 0.05, 0)
     n = len(pos)
@@ -226,18 +223,6 @@ def c1_compare(lib, threads, n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100
 
     cpu_s, cpu_k = timed(cpu_eval)
     cpu_fft_s, _ = timed(cpu_eval_fft)
-    scat = ElasticScatter(dict(C1_EXP), seed=0)
-    atoms = Atoms("Au%d" % n, pos)
-    g_ideal = scat.get_pdf(Atoms("Au%d" % n, ideal))
-
-    def gpu_eval():
-        scat.get_fq(atoms)
-        scat.get_grad_fq(atoms)
-        g = scat.get_pdf(atoms)
-        scat.get_grad_pdf(atoms)
-        return wrap_rw(g, g_ideal)
-
-    gpu_s, gpu_k = timed(gpu_eval)
 
     # exact HMC: momenta ~ N(0, m / beta), velocity Verlet with the force
     # carried between steps, Metropolis on exp(beta (H0 - H1))
@@ -260,12 +245,7 @@ def c1_compare(lib, threads, n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100
         if rs.random_sample() < np.exp(min(0.0, beta * (h0 - h1))):
             x, ex, fx, acc = y, ey, f, acc + 1
     cpu_hmc_s = (time.perf_counter() - t0) / n_prop
-    hmc = DeviceHMC(scat, [(pos, np.full(n, 79))], g_ideal[None, :], conv=conv)
-    hmc.run(2, n_leapfrog=n_leap, step=eps, beta=beta, seed=1)      # capture
-    _, a_dev, t = hmc.run(4 * n_prop, n_leapfrog=n_leap, step=eps, beta=beta, seed=1,
-                          times=True)
-    hmc.close()
-    gpu_hmc_s = float(t[-1]) / 1e3 / (4 * n_prop)
+    dev = c1_device(n_prop, n_leap, eps, beta, conv)
     return {
         "workload": "55-atom Au cuboctahedron (a=4.08 A), sigma=0.05 A seeded jitter; "
                     "Q 0-25 step 0.1, r 0-40 step 0.01; Rw against the ideal particle",
@@ -275,15 +255,55 @@ def c1_compare(lib, threads, n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100
         "cpu_s_per_eval_fft_rows": cpu_fft_s,
         "cpu_threads": "1 for the pair sums (1,485 pairs); BLAS / %d FFT workers for the "
                        "transform" % threads,
-        "gpu_s_per_eval": gpu_s, "gpu_evals": gpu_k,
+        "gpu_s_per_eval": dev["s_per_eval"], "gpu_evals": dev["evals"],
         "gpu_eval_path": "ElasticScatter getters (host arrays, PCIe round trips included)",
         "hmc": {"leapfrog_steps": n_leap, "step": eps, "beta": beta, "conv": conv,
                 "masses": "ASE (Au 196.97 amu)",
                 "cpu_s_per_proposal": cpu_hmc_s, "cpu_proposals": n_prop,
                 "cpu_acceptance": acc / n_prop,
-                "gpu_s_per_proposal": gpu_hmc_s, "gpu_proposals": 4 * n_prop,
-                "gpu_acceptance": float(a_dev.mean())},
+                "gpu_s_per_proposal": dev["s_per_proposal"], "gpu_proposals": dev["proposals"],
+                "gpu_acceptance": dev["acceptance"]},
     }
+
+
+def c1_device(n_prop=10, n_leap=10, eps=2e-3, beta=50.0, conv=100.0, min_s=2.0):
+    """The device half of the 55-atom cuboctahedron comparison: one evaluation
+    of F(Q), grad F, G(r), grad G and Rw through the ElasticScatter getters
+    (host arrays), and exact HMC on the device (4 n_prop proposals after a
+    captured warm-up), seconds per evaluation and per proposal."""
+    from pyiid_amd import Atoms, structures
+    from pyiid_amd.calc import wrap_rw
+    from pyiid_amd.elasticscatter import ElasticScatter
+    from pyiid_amd.sim import DeviceHMC
+    ideal = structures.cuboctahedron(4.08)
+    pos = structures.jitter(ideal, 0.05, 0)
+    n = len(pos)
+    scat = ElasticScatter(dict(C1_EXP), seed=0)
+    atoms = Atoms("Au%d" % n, pos)
+    g_ideal = scat.get_pdf(Atoms("Au%d" % n, ideal))
+
+    def gpu_eval():
+        scat.get_fq(atoms)
+        scat.get_grad_fq(atoms)
+        g = scat.get_pdf(atoms)
+        scat.get_grad_pdf(atoms)
+        return wrap_rw(g, g_ideal)
+
+    gpu_eval()
+    k, t0 = 0, time.perf_counter()
+    while True:
+        gpu_eval()
+        k += 1
+        if time.perf_counter() - t0 >= min_s:
+            break
+    gpu_s = (time.perf_counter() - t0) / k
+    hmc = DeviceHMC(scat, [(pos, np.full(n, 79))], g_ideal[None, :], conv=conv)
+    hmc.run(2, n_leapfrog=n_leap, step=eps, beta=beta, seed=1)      # capture
+    _, a_dev, t = hmc.run(4 * n_prop, n_leapfrog=n_leap, step=eps, beta=beta, seed=1,
+                          times=True)
+    hmc.close()
+    return {"s_per_eval": gpu_s, "evals": k, "s_per_proposal": float(t[-1]) / 1e3 / (4 * n_prop),
+            "proposals": 4 * n_prop, "acceptance": float(a_dev.mean())}
 
 
 def headline_20k(steps=3, warmup=1):
@@ -607,6 +627,9 @@ def main():
                     help="only run the 256-chain sampler timing (for tracing its graphs)")
     ap.add_argument("--only-hmc", action="store_true",
                     help="only run the exact / learned-integrator sampler comparison")
+    ap.add_argument("--only-c1-device", action="store_true",
+                    help="only the device half of the 55-atom cuboctahedron comparison "
+                         "(evaluation through the getters and exact HMC)")
     ap.add_argument("--hmc-proposals", type=int, default=40,
                     help="exact-HMC proposals per seed of the sampler comparison (0 skips it)")
     ap.add_argument("--grad-kernel", choices=("auto", "ordered", "symmetric"), default="auto",
@@ -638,6 +661,9 @@ def main():
         return
     if args.only_hmc:
         print(json.dumps(hmc_comparison()))
+        return
+    if args.only_c1_device:
+        print(json.dumps(c1_device()))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
