@@ -29,8 +29,12 @@
 //    per lane = row): one slot per (item, I) for the I rows (one FP32 run
 //    over the J range, written once) and one per (item, J) for the J rows,
 //    which the item's I tiles add to in turn (the first writes, the others
-//    add with no-return atomics; the item's block is the slot's only writer,
-//    so its adds land in program order).  reduce_slots_kernel sums every
+//    add with no-return atomics).  The item's block is the slot's only
+//    writer, the same thread writes a given float every time, and the FP64
+//    lane reduction at the end of each I tile passes a full barrier (which
+//    waits for the thread's stores and atomics), so a slot's first store has
+//    completed before the next I tile adds to it and the adds land in I
+//    order: the sums repeat bit for bit.  reduce_slots_kernel sums every
 //    tile's slots in FP64.
 //
 // Reference semantics (pyIID): the gradient of F(Q), cpu_flat.py:77-153 and
@@ -98,7 +102,8 @@ template <int S>
 using IC = std::integral_constant<int, S>;
 
 // Block barrier that orders LDS only: the slot stores and atomics in flight
-// need no wait (every slot has one writer, the same thread throughout).
+// need no wait here (every slot float has one writer thread, and the FP64
+// lane reduction of each I tile, which uses a full barrier, orders them).
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
