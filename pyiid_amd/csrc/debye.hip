@@ -8,13 +8,13 @@
 //    with a Cody-Waite split of pi/2; sin/cos come from Taylor polynomials in
 //    FP32.  Along Q, sin(Q r)/r and cos(Q r) follow the three-term
 //    recurrence with 2 cos(dtheta) as a two-float constant (FP64 setup, the
-//    low part applied every fourth bin), so the inner loop is FMA only.
+//    low part applied every eighth bin), so the inner loop is FMA only.
 //    Each lane accumulates F and the three gradient components for its atom;
 //    no atomics, no K x Q intermediates.
 //  * fq_kernel: unordered pairs (j < i) for F(Q) alone; omega_m = sin(Q_m r)/r
 //    follows the three-term recurrence w_{m+1} = 2 cos(dtheta) w_m - w_{m-1}
 //    with 2 cos(dtheta) as a two-float constant, the low part applied every
-//    fourth bin (2.25 VALU ops per pair and Q bin), when every pair of the
+//    eighth bin (2.125 VALU ops per pair and Q bin), when every pair of the
 //    wave has its step at least 0.2 away from a multiple of pi, else the
 //    Reinsch form
 //    w_{m+1} = w_m + D_{m+1}, D_{m+2} = D_{m+1} - eps w_{m+1}, eps =
@@ -50,7 +50,7 @@ constexpr int cmax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b >
 // sums, M = 16 in 8 waves.  Per round of GJ j atoms the block's waves (same
 // atoms, different Q chunks) share the per-pair setup through LDS (FP32,
 // [field][k][lane]): 1/r, (q_i - q_j)/r^2, sin(dtheta), 2 cos(dtheta) as hi
-// and 4 lo, and the anchor sin, cos(Q0 r) of every wave's Q chunk.
+// and 8 lo, and the anchor sin, cos(Q0 r) of every wave's Q chunk.
 // hi + lo -> hi, with the rounding error of that addition left in lo
 // (TwoSum: exact for any magnitudes, no branches).
 __device__ __forceinline__ void two_sum_fold(float& hi, float& lo) {
@@ -76,7 +76,7 @@ struct FusedShape {
                                      kJTile * 32 + GJ * 64 * kFields * 4);
 };
 // F-only kernel: rounds of kFqGeoJ j atoms share the per-pair setup (FP32,
-// [field][k][lane]): 1/r, sin(dtheta), 2 cos(dtheta) as hi and 4 lo, eps of d,
+// [field][k][lane]): 1/r, sin(dtheta), 2 cos(dtheta) as hi and 8 lo, eps of d,
 // (-1)^k (dtheta = k pi + d), and the anchor sin, cos(Q0 r) of each of the
 // block's kWaves Q chunks.  Rounds of 8 keep the block at 36 KB of LDS.
 constexpr int kFqGeoJ = 8;
@@ -203,7 +203,7 @@ __device__ __forceinline__ void fused_grad_body(
           geo_f[o + 3 * kGS] = g.dpz;
           geo_f[o + 4 * kGS] = s1;
           geo_f[o + 5 * kGS] = k2hi;
-          geo_f[o + 6 * kGS] = 4.0f * k2lo;
+          geo_f[o + 6 * kGS] = 8.0f * k2lo;
           geo_f[o + 7 * kGS] = static_cast<int>(pj.w) < i ? 1.0f : 0.0f;  // F weight
         }
         // the anchors sin, cos(Q0 r) of the part's 8 Q chunks: the first
@@ -237,14 +237,14 @@ __device__ __forceinline__ void fused_grad_body(
         auto bins = [&](const float (&g)[10], auto with_f) {
           constexpr bool kWithF = decltype(with_f)::value;
           const float inv_r = g[0], dpx = g[1], dpy = g[2], dpz = g[3], s1 = g[4], k2 = g[5],
-                      lo4 = g[6], fw = g[7];
+                      lo8 = g[6], fw = g[7];
           float sn = g[8], cs = g[9];
           // Along Q, w = sin(Q r) / r and cs = cos(Q r) follow the three-term
           // recurrence y' = 2 cos(dtheta) y - y_prev (one FMA each per bin)
-          // with 2 cos(dtheta) = k2 + lo: the low part is added every fourth
-          // bin as 4 lo, which keeps the frequency exact on average; the
-          // remaining error is bounded (about 50 ulp after 32 bins for close
-          // pairs) instead of the drift of an FP32-rounded coefficient.
+          // with 2 cos(dtheta) = k2 + lo: the low part is added every eighth
+          // bin as 8 lo, which keeps the frequency exact on average; the
+          // remaining error is bounded (about 75 ulp after 32 bins for steps
+          // near k pi) instead of the drift of an FP32-rounded coefficient.
           const float c1 = 0.5f * k2;
           float w = sn * inv_r;
           float wp = fmaf(sn, c1, -cs * s1) * inv_r;  // sin(Q0 r - dtheta) / r
@@ -258,9 +258,9 @@ __device__ __forceinline__ void fused_grad_body(
             sz[m] = fmaf(t, dpz, sz[m]);
             float wn = fmaf(k2, w, -wp);
             float cn = fmaf(k2, cs, -cp);
-            if ((m & 3) == 3) {
-              wn = fmaf(lo4, w, wn);
-              cn = fmaf(lo4, cs, cn);
+            if ((m & 7) == 7) {
+              wn = fmaf(lo8, w, wn);
+              cn = fmaf(lo8, cs, cn);
             }
             wp = w;
             cp = cs;
@@ -368,16 +368,16 @@ __global__ __launch_bounds__(64 * WAVES) void fused_grad_kernel(
 // F-only inner loops over NP j atoms of a round at once (their LDS offsets
 // o[p]), adding sin(Q_m r)/r of each pair to acc[m] for the wave's M bins.
 // Three-term form: w_{m+2} = 2 cos(dtheta) w_{m+1} - w_m with 2 cos(dtheta)
-// as hi + lo, the low part added every fourth bin as 4 lo.
+// as hi + lo, the low part added every eighth bin as 8 lo.
 template <int M, int NP>
 __device__ __forceinline__ void fq_three_term(float (&acc)[M], const float* geo_f, int kGS,
                                               const int (&o)[NP], int chunk) {
-  float w0[NP], w1[NP], k2[NP], lo4[NP];
+  float w0[NP], w1[NP], k2[NP], lo8[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
     const float inv_r = geo_f[o[p]], s1 = geo_f[o[p] + kGS];
     k2[p] = geo_f[o[p] + 2 * kGS];
-    lo4[p] = geo_f[o[p] + 3 * kGS];
+    lo8[p] = geo_f[o[p] + 3 * kGS];
     const float sn = geo_f[o[p] + (6 + 2 * chunk) * kGS], cs = geo_f[o[p] + (7 + 2 * chunk) * kGS];
     w0[p] = sn * inv_r;
     w1[p] = fmaf(sn, 0.5f * k2[p], cs * s1) * inv_r;
@@ -391,7 +391,7 @@ __device__ __forceinline__ void fq_three_term(float (&acc)[M], const float* geo_
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       float w2 = fmaf(k2[p], w1[p], -w0[p]);
-      if ((m & 3) == 3) w2 = fmaf(lo4[p], w1[p], w2);
+      if ((m & 7) == 7) w2 = fmaf(lo8[p], w1[p], w2);
       w0[p] = w1[p];
       w1[p] = w2;
     }
@@ -510,7 +510,7 @@ fq_kernel(const AtomRec* __restrict__ pos, const int32_t* __restrict__ perm,
         const float k2hi = static_cast<float>(k2);
         geo_f[o + kGS] = sgn * 2.0f * sh * ch;
         geo_f[o + 2 * kGS] = k2hi;
-        geo_f[o + 3 * kGS] = 4.0f * static_cast<float>(k2 - static_cast<double>(k2hi));
+        geo_f[o + 3 * kGS] = 8.0f * static_cast<float>(k2 - static_cast<double>(k2hi));
         geo_f[o + 4 * kGS] = static_cast<float>(4.0 * shd * shd);
         geo_f[o + 5 * kGS] = sgn;
         // the anchors sin, cos(Q0 r) of the block's kWaves Q chunks: the
@@ -622,7 +622,7 @@ constexpr int kPfAnchor = 4;  // chunks per FP64-reduced anchor
 // q_lo, with uw[k] = (u, u Q) of bin q_lo + k:
 //   A = sum u Q cos(Q r),  B = sum u sin(Q r).
 // sin and cos follow the three-term recurrence (one FMA each per bin) with
-// 2 cos(dtheta) = k2 + lo, the low part added every fourth bin: exact
+// 2 cos(dtheta) = k2 + lo, the low part added every eighth bin: exact
 // frequency on average, bounded error for every dtheta (close pairs and steps
 // near multiples of pi included), no per-wave branch.  The recurrences restart
 // every kPfChunk bins from an anchor: reduced in FP64 every kPfAnchor chunks,
@@ -632,7 +632,7 @@ template <int NP>
 __device__ __forceinline__ void pf_sums(const float2* uw, int span, int q_lo, double dq,
                                         const PairGeom (&g)[NP], float (&S)[NP]) {
   double dth[NP];
-  float k2[NP], lo4[NP], c1[NP], s1[NP], rs[NP], rc[NP], an_s[NP], an_c[NP];
+  float k2[NP], lo8[NP], c1[NP], s1[NP], rs[NP], rc[NP], an_s[NP], an_c[NP];
   float a0[NP], a1[NP], b0[NP], b1[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
@@ -640,7 +640,7 @@ __device__ __forceinline__ void pf_sums(const float2* uw, int span, int q_lo, do
     float lo;
     two_cos(dth[p], k2[p], lo, s1[p]);
     c1[p] = 0.5f * k2[p];
-    lo4[p] = 4.0f * lo;
+    lo8[p] = 8.0f * lo;
     rs[p] = 0.f;
     rc[p] = 1.f;
     if (span > kPfChunk) sincos_red(static_cast<double>(kPfChunk) * dth[p], rs[p], rc[p]);
@@ -678,9 +678,9 @@ __device__ __forceinline__ void pf_sums(const float2* uw, int span, int q_lo, do
         }
         float sn2 = fmaf(k2[p], sn[p], -sp[p]);
         float cs2 = fmaf(k2[p], cs[p], -cp[p]);
-        if ((m & 3) == 3) {
-          sn2 = fmaf(lo4[p], sn[p], sn2);
-          cs2 = fmaf(lo4[p], cs[p], cs2);
+        if ((m & 7) == 7) {
+          sn2 = fmaf(lo8[p], sn[p], sn2);
+          cs2 = fmaf(lo8[p], cs[p], cs2);
         }
         sp[p] = sn[p];
         cp[p] = cs[p];

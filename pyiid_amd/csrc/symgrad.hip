@@ -52,7 +52,7 @@ constexpr int kSymS = 16;  // steps per phase (one setup round)
 // Launch shapes: M Q bins per wave, W waves per block (M W bins per block):
 // 16 x 8 (2 waves per SIMD) and 12 x 12 (3 per SIMD, within 168 VGPRs).
 // LDS: the setup as FP32 fields [field][step][lane] (1/r, (q_i - q_j)/r^2,
-// sin dtheta, hi, 4 lo, then sin, cos(Q0 r) of each wave's chunk), then the
+// sin dtheta, hi, 8 lo, then sin, cos(Q0 r) of each wave's chunk), then the
 // J tile's atoms.  The FP64 lane reduction reuses the start.
 template <int M, int W>
 struct SymShape {
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
           geo[o + 3 * GS] = g.dpz;
           geo[o + 4 * GS] = s1;
           geo[o + 5 * GS] = k2hi;
-          geo[o + 6 * GS] = 4.0f * k2lo;
+          geo[o + 6 * GS] = 8.0f * k2lo;
           // anchors sin, cos(Q0 r) of the W waves' chunks: the first reduced
           // in FP64, the others by FP32 rotations by M dtheta
           float as, ac, rsn, rcs;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
         // written): a branch around them would split the accumulators'
         // registers between two paths.
         //
-        // g: 1/r, dpx, dpy, dpz, sin dtheta, hi, 4 lo, anchor sin, cos
+        // g: 1/r, dpx, dpy, dpz, sin dtheta, hi, 8 lo, anchor sin, cos
         struct G {
           float f[9];
         };
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
         auto step = [&](auto sc, const G& g) {
           constexpr int S = decltype(sc)::value;
           const float inv_r = g.f[0], dpx = g.f[1], dpy = g.f[2], dpz = g.f[3];
-          const float s1 = g.f[4], k2 = g.f[5], lo4 = g.f[6];
+          const float s1 = g.f[4], k2 = g.f[5], lo8 = g.f[6];
           const float px = row_ror<S>(dpx), py = row_ror<S>(dpy), pz = row_ror<S>(dpz);
           const float sn = g.f[7];
           float cs = g.f[8];
@@ -262,9 +262,9 @@ __global__ __launch_bounds__(64 * W) void sym_grad_kernel(
             iz[m] = fmaf(t[m], dpz, iz[m]);
             float wn = fmaf(k2, w, -wp);
             float cn = fmaf(k2, cs, -cp);
-            if ((m & 3) == 3) {
-              wn = fmaf(lo4, w, wn);
-              cn = fmaf(lo4, cs, cn);
+            if ((m & 7) == 7) {
+              wn = fmaf(lo8, w, wn);
+              cn = fmaf(lo8, cs, cn);
             }
             wp = w;
             cp = cs;
