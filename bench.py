@@ -379,8 +379,9 @@ def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10):
     sampler's time to target the HIP-event time at which the chains' mean
     Rw first reaches it.  The learned integrator trains every train_every
     proposals and falls back to exact forces for a block when the learned
-    block before it accepted under 30% (the time includes its training
-    steps and its exact blocks).  Both samplers start from the same
+    block before it, or the block's own first three learned proposals,
+    accepted under 30% (the time includes its training steps and its exact
+    proposals).  Both samplers start from the same
     structures."""
     from pyiid_amd import workloads
     from pyiid_amd.elasticscatter import ElasticScatter

@@ -291,7 +291,9 @@ int pyiid_sur_train(pyiid_ctx *ctx, double forget, double ridge, double *r2, int
  * when the mean r2 of the training step is below min_r2, or when the
  * learned-force proposals since the previous step were accepted less often
  * than min_acceptance (0 disables the test; after an exact block the learned
- * forces are tried again).  energies / accepted /
+ * forces are tried again).  With train_every > 3 the acceptance is also
+ * checked after a block's first three learned-force proposals, and a block
+ * that falls short there finishes with exact forces.  energies / accepted /
  * exact_mode (per proposal) / times_ms as in pyiid_hmc_run; violations
  * (n_chain int32, nullable): proposals rejected for leaving the neighbour
  * lists' range, counted since the layout was set. */

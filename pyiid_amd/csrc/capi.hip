@@ -1292,6 +1292,8 @@ int hmc_proposal(pyiid_ctx* c, int n_leapfrog, double eps, double beta, uint64_t
 // ---- learned integrator ----
 
 constexpr int kGramChunk = 2048;  // flat force columns per Gram partial
+// learned-force proposals after which a block's acceptance is first checked
+constexpr int kSurAcceptProbe = 3;
 
 uint16_t host_bf16(float x) {
   uint32_t u;
@@ -2453,25 +2455,32 @@ int pyiid_sur_hmc_run(pyiid_ctx* c, int n_proposals, int n_leapfrog, double step
   bool exact_block = false;
   int block_start = 0;
   std::vector<int32_t> acc_h;
+  // true when proposals [block_start, p) were accepted less often than min_acceptance
+  auto low_acceptance = [&](int p, bool* low) -> int {
+    acc_h.resize(static_cast<size_t>(p - block_start) * B);
+    HIPCHK(hipMemcpyAsync(acc_h.data(), h.acchist.as<int32_t>() + (size_t)block_start * B,
+                          sizeof(int32_t) * acc_h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int64_t n_acc = 0;
+    for (int32_t a : acc_h) n_acc += a;
+    *low = static_cast<double>(n_acc) < min_acceptance * static_cast<double>(acc_h.size());
+    return PYIID_OK;
+  };
   for (int p = 0; p < n_proposals; ++p) {
     if (!s.w3_set || (train_every > 0 && p % train_every == 0)) {
       // acceptance of the learned-force block that just ended
       bool low_acc = false;
-      if (min_acceptance > 0.0 && p > block_start && !exact_block) {
-        acc_h.resize(static_cast<size_t>(p - block_start) * B);
-        HIPCHK(hipMemcpyAsync(acc_h.data(), h.acchist.as<int32_t>() + (size_t)block_start * B,
-                              sizeof(int32_t) * acc_h.size(), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        int64_t n_acc = 0;
-        for (int32_t a : acc_h) n_acc += a;
-        low_acc = static_cast<double>(n_acc) < min_acceptance * static_cast<double>(acc_h.size());
-      }
+      if (min_acceptance > 0.0 && p > block_start && !exact_block) CHK(low_acceptance(p, &low_acc));
       CHK(sur_train_step(c, forget, ridge));
       double mean = 0.0;
       for (int b = 0; b < B; ++b) mean += s.r2[b];
       // poor fit, or learned proposals mostly rejected: exact forces until the next fit
       exact_block = mean / B < min_r2 || low_acc;
       block_start = p;
+    } else if (min_acceptance > 0.0 && !exact_block && train_every > kSurAcceptProbe &&
+               p - block_start == kSurAcceptProbe) {
+      // the block's first learned proposals already fall short: exact forces for the rest of it
+      CHK(low_acceptance(p, &exact_block));
     }
     if (exact_block) CHK(refresh_exact(c));
     CHK(run_proposal(c, !exact_block, use_graph != 0, n_leapfrog, step, beta, seed));

@@ -218,7 +218,9 @@ class SurrogateHMC(DeviceHMC):
         (proposals run with exact forces), times_ms, violations.  A block of
         train_every proposals runs with exact forces when the fit before it
         has mean r2 < min_r2 or the learned-force block before it accepted
-        less often than min_acceptance."""
+        less often than min_acceptance; with train_every > 3 a block whose
+        first three learned-force proposals fall short of min_acceptance
+        finishes with exact forces."""
         return self.ctx.sur_hmc_run(n_proposals, n_leapfrog, step, beta, seed, self.B, use_graph,
                                     train_every, forget, ridge, min_r2, min_acceptance)
 

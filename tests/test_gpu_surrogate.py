@@ -227,16 +227,21 @@ def test_neighbour_capacity_is_checked():
         _sampler(chains, max_nbr=8)
 
 
-def test_low_acceptance_switches_the_next_block_to_exact_forces():
-    """min_acceptance: a learned-force block accepted less often than the bar
-    is followed by a block of exact-force proposals, after which the learned
-    forces are tried again.  A bar above 1 fails every learned block, so the
-    blocks alternate; a bar of 0 never switches."""
+def test_low_acceptance_switches_to_exact_forces():
+    """min_acceptance: a block whose first three learned-force proposals were
+    accepted less often than the bar runs the rest of the block with exact
+    forces, and the next block tries the learned forces again.  Blocks of
+    three or fewer proposals are judged when they end: a failed one is
+    followed by a block of exact forces.  A bar above 1 fails every learned
+    proposal; a bar of 0 never switches."""
     chains = _chains([24, 30], seed=11)
     hmc, _, _, _ = _sampler(chains, conv=50.0)
     out = hmc.run(30, n_leapfrog=3, step=0.001, beta=5.0, seed=2, train_every=10,
                   min_acceptance=1.01)
-    np.testing.assert_array_equal(out["exact_mode"], [0] * 10 + [1] * 10 + [0] * 10)
+    np.testing.assert_array_equal(out["exact_mode"], ([0] * 3 + [1] * 7) * 3)
+    out = hmc.run(12, n_leapfrog=3, step=0.001, beta=5.0, seed=4, train_every=3,
+                  min_acceptance=1.01)
+    np.testing.assert_array_equal(out["exact_mode"], ([0] * 3 + [1] * 3) * 2)
     out = hmc.run(20, n_leapfrog=3, step=0.001, beta=5.0, seed=3, train_every=10,
                   min_acceptance=0.0)
     np.testing.assert_array_equal(out["exact_mode"], [0] * 20)
