@@ -8,8 +8,10 @@ and getters).  Differences, by design:
 * the only processor is ``'MI355X'`` (HIP kernels).  ``set_processor`` raises
   if it is unavailable or another processor is requested, instead of silently
   keeping the previous one (``__init__.py:228-292``);
-* per-element form factors come from the Cromer-Mann table
-  (:mod:`pyiid_amd.formfactors`) instead of ``xraylib``;
+* per-element form factors come from ``xraylib.FF_Rayl`` as in the
+  reference when xraylib is installed, and from the Cromer-Mann table
+  otherwise (``form_factors=`` "auto", "xraylib" or "cromer-mann";
+  :mod:`pyiid_amd.formfactors`);
 * with ``compat_formfactors=False`` (default) each element's row is evaluated
   for that element on its own grid.  ``compat_formfactors=True`` reproduces
   the reference's ``_wrap_atoms`` indexing (rows taken for
@@ -21,7 +23,7 @@ import math
 import numpy as np
 
 from . import processor as _proc
-from .formfactors import element_table, form_factor
+from .formfactors import element_table, form_factor, resolve_source
 
 all_changes = ["positions", "numbers", "cell", "pbc", "charges", "magmoms", "exp"]
 
@@ -49,8 +51,10 @@ class ElasticScatter(object):
     """Elastic powder scattering of a set of atoms: F(Q), S(Q), I(Q), G(r) and
     the gradients of F(Q) and G(r)."""
 
-    def __init__(self, exp_dict=None, verbose=False, seed=None, compat_formfactors=False):
+    def __init__(self, exp_dict=None, verbose=False, seed=None, compat_formfactors=False,
+                 form_factors="auto"):
         self.verbose = verbose
+        self.form_factor_source = resolve_source(form_factors)
         self.wrap_atoms_state = None
         if seed is None:
             self.seed = int(np.random.random() * 2 ** 32)
@@ -87,13 +91,14 @@ class ElasticScatter(object):
             if self.compat_formfactors:
                 e_list = list(set(numbers.tolist()))
                 q = np.arange(nq) * self.exp["qbin"]
-                rows = [form_factor(numbers[i], q) for i in range(len(e_list))]
+                rows = [form_factor(numbers[i], q, self.form_factor_source)
+                        for i in range(len(e_list))]
                 table = np.asarray(rows, dtype=np.float32).reshape(len(e_list), nq)
                 index = np.zeros(n, dtype=np.int32)
                 for i, z in enumerate(e_list):
                     index[numbers == z] = i
             else:
-                table, index, _ = element_table(numbers, qbin, nq)
+                table, index, _ = element_table(numbers, qbin, nq, self.form_factor_source)
             per_atom = table[index] if n else np.zeros((0, nq), np.float32)
             if name in atoms.arrays:
                 del atoms.arrays[name]

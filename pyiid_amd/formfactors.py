@@ -1,17 +1,24 @@
-"""Per-element X-ray form factors f(Q) from the Cromer-Mann four-Gaussian fit.
+"""Per-element X-ray form factors f(Q).
 
 The reference evaluates ``xraylib.FF_Rayl(Z, Q / 4 pi)`` per element and bin
-(``pyiid/experiments/elasticscatter/kernels/master_kernel.py:14-36``).  xraylib
-is not available here; this module provides the International Tables
-(Vol. C, Table 6.1.1.4) Cromer-Mann parameterisation
+(``pyiid/experiments/elasticscatter/kernels/master_kernel.py:14-36``).  Two
+sources are provided:
 
-    f(s) = sum_{i=1..4} a_i exp(-b_i s^2) + c,   s = sin(theta)/lambda = Q / (4 pi)
+* ``"xraylib"``: that same call, used when xraylib is installed;
+* ``"cromer-mann"``: the International Tables (Vol. C, Table 6.1.1.4)
+  Cromer-Mann parameterisation
 
-for the elements used by the workloads.  For a single-element structure the
-table cancels exactly against the normalisation, so F(Q), G(r) and their
-gradients do not depend on these values.
+      f(s) = sum_{i=1..4} a_i exp(-b_i s^2) + c,   s = sin(theta)/lambda = Q / (4 pi)
+
+  for the elements used by the workloads, with no dependency.
+
+``"auto"`` picks xraylib when it can be imported.  For a single-element
+structure the table cancels exactly against the normalisation, so F(Q), G(r)
+and their gradients do not depend on the source.
 """
 import numpy as np
+
+SOURCES = ("auto", "xraylib", "cromer-mann")
 
 # Z: (symbol, (a1, b1, a2, b2, a3, b3, a4, b4, c))
 CROMER_MANN = {
@@ -43,8 +50,39 @@ MASSES = {6: 12.011, 8: 15.999, 14: 28.085, 26: 55.845, 28: 58.6934, 29: 63.546,
           47: 107.8682, 78: 195.084, 79: 196.966569}
 
 
-def form_factor(z, q):
-    """f(Q) of element ``z`` on the Q values ``q`` (A^-1), float64."""
+def _xraylib():
+    """The xraylib module when it is installed, else None."""
+    try:
+        import xraylib
+    except ImportError:
+        return None
+    return xraylib
+
+
+def resolve_source(source):
+    """The concrete source for ``source``: "auto" becomes "xraylib" when it
+    can be imported and "cromer-mann" otherwise.  Raises ValueError for an
+    unknown name and ImportError for "xraylib" without xraylib."""
+    if source not in SOURCES:
+        raise ValueError("form factor source must be one of %s, got %r" % (SOURCES, source))
+    if source == "auto":
+        return "xraylib" if _xraylib() is not None else "cromer-mann"
+    if source == "xraylib" and _xraylib() is None:
+        raise ImportError("form factor source 'xraylib' requested but xraylib is not installed")
+    return source
+
+
+def form_factor(z, q, source="cromer-mann"):
+    """f(Q) of element ``z`` on the Q values ``q`` (A^-1), float64, from
+    ``source`` (see ``resolve_source``)."""
+    source = resolve_source(source)
+    if source == "xraylib":
+        xrl = _xraylib()
+        # xraylib takes s = sin(theta)/lambda = Q / (4 pi), divided in the
+        # reference's order (master_kernel.py:35-36)
+        return np.array([xrl.FF_Rayl(int(z), float(qk) / 4 / np.pi)
+                         for qk in np.asarray(q, dtype=np.float64).ravel()],
+                        dtype=np.float64).reshape(np.shape(q))
     try:
         _, p = CROMER_MANN[int(z)]
     except KeyError:
@@ -57,7 +95,7 @@ def form_factor(z, q):
     return f
 
 
-def element_table(numbers, qbin, nq):
+def element_table(numbers, qbin, nq, source="cromer-mann"):
     """Table f[e][m] (float32, E x nq) on Q_m = m * qbin for the elements of
     ``numbers`` in ascending Z order, plus the per-atom element index."""
     numbers = np.asarray(numbers)
@@ -65,5 +103,5 @@ def element_table(numbers, qbin, nq):
     q = np.arange(nq, dtype=np.float64) * qbin
     if len(elems) == 0:
         return np.ones((1, nq), np.float32), index.astype(np.int32), elems
-    table = np.stack([form_factor(z, q) for z in elems]).astype(np.float32)
+    table = np.stack([form_factor(z, q, source) for z in elems]).astype(np.float32)
     return table, index.astype(np.int32), elems

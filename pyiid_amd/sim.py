@@ -159,7 +159,8 @@ class DeviceHMC(object):
         self.sizes = np.array([len(p) for p in pos])
         self.offsets = self.ctx.set_batch(pos)
         nq = int(np.floor(scat.exp["qmax"] / scat.pdf_qbin))
-        table, index, self.elements = element_table(numbers, scat.pdf_qbin, nq)
+        table, index, self.elements = element_table(numbers, scat.pdf_qbin, nq,
+                                                     scat.form_factor_source)
         self.ctx.set_elements(index)
         self.ctx.set_scatter(_lib.SLOT_PDF, table)
         r = scat.get_r()

@@ -50,7 +50,7 @@ def target_pdfs(scat, target_positions, numbers, device=0):
     ctx.set_batch(target_positions)
     nq = int(np.floor(scat.exp["qmax"] / scat.pdf_qbin))
     allz = np.concatenate(numbers)
-    table, index, _ = element_table(allz, scat.pdf_qbin, nq)
+    table, index, _ = element_table(allz, scat.pdf_qbin, nq, scat.form_factor_source)
     ctx.set_elements(index)
     ctx.set_scatter(_lib.SLOT_PDF, table)
     fq = ctx.fq(_lib.SLOT_PDF, scat.pdf_qbin, nq, n_chain=len(target_positions))

@@ -39,3 +39,64 @@ def test_element_table_layout():
     np.testing.assert_array_equal(index, [2, 0, 2, 1, 0])
     np.testing.assert_allclose(table[index[0]], formfactors.form_factor(79, np.arange(7) * 0.1),
                                rtol=1e-6)
+
+
+def _fake_xraylib(monkeypatch, calls):
+    """A stand-in xraylib module whose FF_Rayl(Z, s) = Z exp(-s) records its
+    arguments, so the plumbing of the xraylib source is testable without it."""
+    import sys
+    import types
+    mod = types.ModuleType("xraylib")
+
+    def ff_rayl(z, s):
+        calls.append((z, s))
+        return z * np.exp(-s)
+
+    mod.FF_Rayl = ff_rayl
+    monkeypatch.setitem(sys.modules, "xraylib", mod)
+
+
+def test_source_resolution(monkeypatch):
+    import sys
+    monkeypatch.setitem(sys.modules, "xraylib", None)   # import raises ImportError
+    assert formfactors.resolve_source("auto") == "cromer-mann"
+    with pytest.raises(ImportError):
+        formfactors.resolve_source("xraylib")
+    with pytest.raises(ValueError):
+        formfactors.resolve_source("waasmaier")
+    _fake_xraylib(monkeypatch, [])
+    assert formfactors.resolve_source("auto") == "xraylib"
+    assert formfactors.resolve_source("cromer-mann") == "cromer-mann"
+
+
+def test_xraylib_source_calls_ff_rayl_at_q_over_4pi(monkeypatch):
+    # the reference fills f[atom][k] = FF_Rayl(Z, k qbin / 4 / pi)
+    # (master_kernel.py:31-36)
+    calls = []
+    _fake_xraylib(monkeypatch, calls)
+    table, index, elems = formfactors.element_table(np.array([79, 28, 79]), 0.1, 5, "xraylib")
+    assert list(elems) == [28, 79]
+    want = np.array([[z * np.exp(-(k * 0.1 / 4 / np.pi)) for k in range(5)] for z in (28, 79)],
+                    np.float32)
+    np.testing.assert_array_equal(table, want)
+    assert [z for z, _ in calls] == [28] * 5 + [79] * 5
+    assert [s for _, s in calls[:5]] == [k * 0.1 / 4 / np.pi for k in range(5)]
+    np.testing.assert_array_equal(index, [1, 0, 1])
+
+
+def test_elasticscatter_uses_the_resolved_source(monkeypatch):
+    from pyiid_amd import Atoms, processor
+    from pyiid_amd import elasticscatter as es
+    monkeypatch.setattr(processor, "device_available", lambda: True)
+    _fake_xraylib(monkeypatch, [])
+    atoms = Atoms(numbers=[28, 79], positions=np.eye(3)[:2])
+    s = es.ElasticScatter({"qmax": 1.0, "qbin": 0.25})
+    assert s.form_factor_source == "xraylib"
+    s._wrap_atoms(atoms)
+    got = atoms.get_array("F(Q) scatter")
+    q = np.arange(4) * 0.25
+    np.testing.assert_array_equal(got[1], (79 * np.exp(-(q / 4 / np.pi))).astype(np.float32))
+    c = es.ElasticScatter({"qmax": 1.0, "qbin": 0.25}, form_factors="cromer-mann")
+    c._wrap_atoms(atoms)
+    np.testing.assert_array_equal(atoms.get_array("F(Q) scatter")[1],
+                                  formfactors.form_factor(79, q).astype(np.float32))
