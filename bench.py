@@ -368,7 +368,7 @@ def _first_at_or_below(rw, target):
     return int(hit[0]) if hit.size else None
 
 
-def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10):
+def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10, energy_weight=None):
     """Wall-seconds to a target Rw: exact-force HMC against the learned
     integrator on 64 chains of 5,000-atom Au particles refined against the
     G(r) of independently rattled copies.  ASE masses (Au 196.97 amu),
@@ -401,7 +401,8 @@ def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10):
         drift = float(rw_e[-20:-10].mean() - plateau)
         target = plateau + 0.005
         k_e = _first_at_or_below(rw_e, target)
-        sur = SurrogateHMC(scat, structs, gt, conv=conv, net_seed=seed)
+        kw = {} if energy_weight is None else {"energy_weight": energy_weight}
+        sur = SurrogateHMC(scat, structs, gt, conv=conv, net_seed=seed, **kw)
         out = sur.run(n_sur, n_leapfrog=n_leap, step=eps, beta=beta, seed=seed,
                       train_every=train_every, forget=0.5, ridge=1e-6, min_r2=0.5,
                       min_acceptance=0.3)

@@ -277,11 +277,24 @@ int pyiid_sur_forces(pyiid_ctx *ctx, double *forces, float *atom_energy, int fla
 /* Feature forces: row k (k < 64) holds the surrogate forces with w3 = e_k
  * for every chain, 64 x (3 n_total) floats. */
 int pyiid_sur_features(pyiid_ctx *ctx, float *phi, int flags);
-/* One training step at the current positions: exact energies and forces
- * (Debye path, as in pyiid_hmc_setup), the normal equations of
- * min |Phi_b w - F_b|^2 added to each chain's running sums after scaling those
- * by `forget`, ridge solve (lambda = ridge * trace / 64) for w3.  r2 (nullable,
- * n_chain doubles): 1 - |F - Phi w|^2 / |F|^2 on this sample. */
+/* Feature energies: e[b][k] = sum over chain b's atoms of the network
+ * output with w3 = e_k (the chain energies of the 64 output units),
+ * n_chain x 64 doubles. */
+int pyiid_sur_feature_energies(pyiid_ctx *ctx, double *e, int flags);
+/* The output weights in use (n_chain x 64 floats, the BF16 values). */
+int pyiid_sur_get_output(pyiid_ctx *ctx, float *w3, int flags);
+/* Weight lambda (A^-2, >= 0; default 1) of the energy rows of the fit (see
+ * pyiid_sur_train); 0 fits forces only. */
+int pyiid_sur_set_energy_weight(pyiid_ctx *ctx, double weight);
+/* One training step at the current positions: exact energies E_b and forces
+ * F_b (Debye path, as in pyiid_hmc_setup), feature forces Phi_b and feature
+ * energies e_b.  Each chain's running normal equations are scaled by
+ * `forget`, then take the force rows min |Phi_b w - F_b|^2 and, when an
+ * earlier training step ran under the same targets and layout, the energy
+ * row lambda |(e_b - e_b') w - (E_b - E_b')|^2 against that step's e_b', E_b'
+ * (the exact Rw change between the two states).  Ridge solve
+ * (ridge * trace / 64) for w3.  r2 (nullable, n_chain doubles):
+ * 1 - |F - Phi w|^2 / |F|^2 on this sample's forces. */
 int pyiid_sur_train(pyiid_ctx *ctx, double forget, double ridge, double *r2, int flags);
 /* n_proposals proposals driven by the surrogate: forces from the network,
  * velocity Verlet and momenta as in pyiid_hmc_run, acceptance with the exact

@@ -14,10 +14,13 @@ device kernels:
   function, pinned by finite differences in the tests.
 * ``forces``: F_a = -sum_j sum_m (c_a,m + c_j,m) g'_m(r_aj) (x_a - x_j) / r_aj.
 * ``ridge_fit``: the output-layer fit (normal equations plus ridge).
+* ``fit_over_states``: the fit over successive training states with
+  forgetting and the energy-difference rows (e_t - e_{t-1}) . w = E_t - E_{t-1}.
 
 Pins: the force formula against central differences of the float64 energy;
 descriptor invariances (translation, rotation, permutation) and the
-single-pair closed form.
+single-pair closed form; ``fit_over_states`` against a QR least-squares
+solve of the stacked, weighted rows and against recovery of a known w.
 """
 import numpy as np
 
@@ -131,3 +134,31 @@ def ridge_fit(phi, y, ridge):
     w = np.linalg.solve(a + lam * np.eye(a.shape[0]), phi @ y)
     res = y - phi.T @ w
     return w, 1.0 - np.dot(res, res) / np.dot(y, y)
+
+
+def fit_over_states(samples, forget, ridge, weight):
+    """Output-layer weights after training steps at successive states.
+
+    samples: per state, in order, (phi (64 x m), y (m), e (64), E): feature
+    forces, exact forces, feature energies and exact energy.  The normal
+    equations are scaled by ``forget`` at every state, then take the state's
+    force rows and, from the second state on, the energy row
+    sqrt(weight) (e_t - e_{t-1}) . w = sqrt(weight) (E_t - E_{t-1}).  Ridge
+    lam = ridge * trace / 64 as in ``ridge_fit``."""
+    a = np.zeros((64, 64))
+    r = np.zeros(64)
+    prev = None
+    for phi, y, e, E in samples:
+        phi = np.asarray(phi, np.float64)
+        a *= forget
+        r *= forget
+        a += phi @ phi.T
+        r += phi @ np.asarray(y, np.float64)
+        e = np.asarray(e, np.float64)
+        if prev is not None and weight > 0:
+            de = e - prev[0]
+            a += weight * np.outer(de, de)
+            r += weight * de * (E - prev[1])
+        prev = (e, E)
+    lam = max(ridge, 1e-12) * np.trace(a) / 64
+    return np.linalg.solve(a + lam * np.eye(64), r)

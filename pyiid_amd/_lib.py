@@ -86,6 +86,9 @@ _SIGNATURES = [
     ("pyiid_sur_descriptors", [_P, _P, _I]),
     ("pyiid_sur_forces", [_P, _P, _P, _I]),
     ("pyiid_sur_features", [_P, _P, _I]),
+    ("pyiid_sur_feature_energies", [_P, _P, _I]),
+    ("pyiid_sur_get_output", [_P, _P, _I]),
+    ("pyiid_sur_set_energy_weight", [_P, _D]),
     ("pyiid_sur_train", [_P, _D, _D, _P, _I]),
     ("pyiid_sur_hmc_run", [_P, _I, _I, _D, _D, ctypes.c_uint64, _I, _I, _D, _D, _D, _D, _P, _P, _P,
                            _P, _P, _I]),
@@ -412,6 +415,19 @@ class Context:
         phi = np.empty((64, 3 * n), np.float32)
         check(self.lib.pyiid_sur_features(self.handle, _ptr(phi), 0))
         return phi
+
+    def sur_feature_energies(self, n_chain):
+        e = np.empty((n_chain, 64), np.float64)
+        check(self.lib.pyiid_sur_feature_energies(self.handle, _ptr(e), 0))
+        return e
+
+    def sur_get_output(self, n_chain):
+        w = np.empty((n_chain, 64), np.float32)
+        check(self.lib.pyiid_sur_get_output(self.handle, _ptr(w), 0))
+        return w
+
+    def sur_set_energy_weight(self, weight):
+        check(self.lib.pyiid_sur_set_energy_weight(self.handle, float(weight)))
 
     def sur_train(self, n_chain, forget=0.0, ridge=1e-6):
         r2 = np.empty(n_chain, np.float64)

@@ -148,11 +148,16 @@ struct Sur {
   float skin = 0.5f;
   int max_nbr = 0;
   Buf net, w3, onehot, nbr, cnt, d, cbuf, eatom, fsur, phi, gpart, gout, viol, viol_count,
-      atom_chain, tiles64, tiles256, chunks, granges;
+      atom_chain, tiles64, tiles256, chunks, granges, efeat, offs, efe;
   int n_t64 = 0, n_t256 = 0, n_chunks = 0;
   uint64_t stamp = ~0ull;
   std::vector<double> acc;  // per chain: accumulated Phi Phi^T, Phi y, y.y (FP64)
   std::vector<double> r2;   // fit quality of the last training sample per chain
+  // energy-difference rows: weight (A^-2) and the previous training state's
+  // feature energies (B x 64) and exact energies (B)
+  double energy_weight = 1.0;
+  bool eprev_set = false;
+  std::vector<double> eprev, Eprev;
 };
 
 }  // namespace
@@ -1346,6 +1351,9 @@ int ensure_sur_layout(pyiid_ctx* c) {
   CHK(s.viol.reserve(sizeof(int32_t) * B));
   CHK(s.viol_count.reserve(sizeof(int32_t) * B));
   CHK(s.w3.reserve(sizeof(uint16_t) * kSurHidden * B));
+  CHK(s.efeat.reserve(sizeof(float) * kSurHidden * n1));
+  CHK(s.efe.reserve(sizeof(double) * kSurHidden * B));
+  CHK(upload_vec(s.offs, c->off));
   if (!t64.empty())
     HIPCHK(hipMemcpy(s.tiles64.p, t64.data(), sizeof(Item) * t64.size(), hipMemcpyHostToDevice));
   if (!t256.empty())
@@ -1362,6 +1370,7 @@ int ensure_sur_layout(pyiid_ctx* c) {
   s.acc.assign(static_cast<size_t>(kSurGram) * B, 0.0);
   s.r2.assign(B, 0.0);
   s.w3_set = false;
+  s.eprev_set = false;
   s.stamp = c->struct_stamp;
   return PYIID_OK;
 }
@@ -1407,7 +1416,8 @@ int sur_check_lists(pyiid_ctx* c) {
 }
 
 // Feature forces Phi[k][3i + c] = -d/dx_ic sum_a h2_k(D_a) for the 64
-// output units (the forces of the network with w3 = e_k).
+// output units (the forces of the network with w3 = e_k), and the feature
+// energies efe[b][k] = sum_{a in chain b} h2_k(D_a) (its chain energies).
 int sur_features(pyiid_ctx* c) {
   Sur& s = c->sur;
   const int64_t n3 = 3 * c->n;
@@ -1418,7 +1428,9 @@ int sur_features(pyiid_ctx* c) {
                             s.nbr.as<int>(), s.cnt.as<int>(), s.rbf, s.d.as<float>()));
   for (int k = 0; k < kSurHidden; ++k)
     CHK(sur_eval(c, s.onehot.as<uint16_t>() + kSurHidden * k, 0, nullptr,
-                 s.phi.as<float>() + k * n3, nullptr, false));
+                 s.phi.as<float>() + k * n3, s.efeat.as<float>() + k * c->n, false));
+  HIPCHK(launch_chain_feature_energy(c->stream, s.efeat.as<float>(), c->n, s.offs.as<int64_t>(),
+                                     c->n_chain, s.efe.as<double>()));
   return PYIID_OK;
 }
 
@@ -1450,8 +1462,13 @@ bool cholesky_solve(std::vector<double>& a, std::vector<double>& r, int n) {
 }
 
 // One training step at the current state: exact energy and forces (they also
-// refresh the sampler's state), feature forces, the chains' normal equations
-// added to the running ones (decayed by `forget`), ridge solve for w3.
+// refresh the sampler's state), feature forces and energies, the chains'
+// normal equations added to the running ones (decayed by `forget`), ridge
+// solve for w3.  The normal equations hold the force rows Phi w = F of this
+// state and, from the second state on, one energy row per chain,
+// sqrt(lambda) (e - e_prev) w = sqrt(lambda) (E - E_prev): the exact energy
+// change since the previous training state, which fixes the network's
+// energy up to the constant it cannot see.
 int sur_train_step(pyiid_ctx* c, double forget, double ridge) {
   Hmc& h = c->hmc;
   Sur& s = c->sur;
@@ -1466,16 +1483,32 @@ int sur_train_step(pyiid_ctx* c, double forget, double ridge) {
                      s.chunks.as<Item>(), s.n_chunks, s.gpart.as<float>(), s.granges.as<Range>(),
                      B, s.gout.as<double>()));
   std::vector<double> g(static_cast<size_t>(kSurGram) * B);
+  std::vector<double> ef(static_cast<size_t>(kSurHidden) * B), en(B);
   HIPCHK(hipMemcpyAsync(g.data(), s.gout.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost,
                         c->stream));
+  HIPCHK(hipMemcpyAsync(ef.data(), s.efe.p, sizeof(double) * ef.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipMemcpyAsync(en.data(), h.ecur.p, sizeof(double) * B, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   constexpr int H = kSurHidden;
+  const bool erow = s.eprev_set && s.energy_weight > 0.0;
   std::vector<uint16_t> w3h(static_cast<size_t>(H) * B);
   std::vector<double> a(H * H), w(H);
   for (int b = 0; b < B; ++b) {
     double* acc = &s.acc[static_cast<size_t>(b) * kSurGram];
     const double* gn = &g[static_cast<size_t>(b) * kSurGram];
     for (int e = 0; e < kSurGram; ++e) acc[e] = forget * acc[e] + gn[e];
+    if (erow) {
+      const double* e1 = &ef[static_cast<size_t>(b) * H];
+      const double* e0 = &s.eprev[static_cast<size_t>(b) * H];
+      const double dE = en[b] - s.Eprev[b], lam = s.energy_weight;
+      for (int k = 0; k < H; ++k) {
+        const double dk = e1[k] - e0[k];
+        for (int l = 0; l < H; ++l) acc[k * H + l] += lam * dk * (e1[l] - e0[l]);
+        acc[H * H + k] += lam * dk * dE;
+      }
+      acc[H * H + H] += lam * dE * dE;
+    }
     double tr = 0.0;
     for (int k = 0; k < H; ++k) tr += acc[k * H + k];
     std::fill(w.begin(), w.end(), 0.0);
@@ -1503,6 +1536,9 @@ int sur_train_step(pyiid_ctx* c, double forget, double ridge) {
   }
   HIPCHK(hipMemcpy(s.w3.p, w3h.data(), sizeof(uint16_t) * w3h.size(), hipMemcpyHostToDevice));
   s.w3_set = true;
+  s.eprev.swap(ef);
+  s.Eprev.swap(en);
+  s.eprev_set = true;
   return PYIID_OK;
 }
 
@@ -2286,6 +2322,7 @@ int pyiid_hmc_setup(pyiid_ctx* c, int slot, double qbin, const double* gobs, int
   HIPCHK(launch_energy_from_out5(c->stream, c->out5.as<double>(), B, conv, h.ecur.as<double>()));
   HIPCHK(hipStreamSynchronize(c->stream));  // `ch` is a host temporary
   h.set = true;
+  c->sur.eprev_set = false;  // a new energy function: no energy difference to the last fit
   h.fcur_stale = false;
   c->sur.stamp = ~0ull;  // a new target: the surrogate fit starts again
   return PYIID_OK;
@@ -2421,6 +2458,40 @@ int pyiid_sur_features(pyiid_ctx* c, float* phi, int flags) {
   CHK(sur_features(c));
   CHK(download(c, phi, c->sur.phi.p, sizeof(float) * kSurHidden * 3 * c->n, flags));
   return finish(c, flags);
+}
+
+int pyiid_sur_feature_energies(pyiid_ctx* c, double* e, int flags) {
+  CHK(sur_ready(c));
+  if (!e) return fail(PYIID_E_INVALID, "null output");
+  CHK(sur_features(c));
+  CHK(download(c, e, c->sur.efe.p, sizeof(double) * kSurHidden * c->n_chain, flags));
+  return finish(c, flags);
+}
+
+int pyiid_sur_get_output(pyiid_ctx* c, float* w3, int flags) {
+  CHK(sur_ready(c));
+  if (!w3) return fail(PYIID_E_INVALID, "null output");
+  if (!c->sur.w3_set) return fail(PYIID_E_STATE, "no output layer has been fitted or set");
+  std::vector<uint16_t> h(static_cast<size_t>(kSurHidden) * c->n_chain);
+  HIPCHK(hipMemcpy(h.data(), c->sur.w3.p, sizeof(uint16_t) * h.size(), hipMemcpyDeviceToHost));
+  std::vector<float> w(h.size());
+  for (size_t e = 0; e < h.size(); ++e) {
+    const uint32_t u = static_cast<uint32_t>(h[e]) << 16;
+    std::memcpy(&w[e], &u, 4);
+  }
+  if (flags & PYIID_DEVICE)
+    HIPCHK(hipMemcpy(w3, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
+  else
+    std::memcpy(w3, w.data(), sizeof(float) * w.size());
+  return PYIID_OK;
+}
+
+int pyiid_sur_set_energy_weight(pyiid_ctx* c, double weight) {
+  CHK(check_ctx(c));
+  if (!(weight >= 0.0) || !std::isfinite(weight))
+    return fail(PYIID_E_INVALID, "energy weight must be finite and >= 0");
+  c->sur.energy_weight = weight;
+  return PYIID_OK;
 }
 
 int pyiid_sur_train(pyiid_ctx* c, double forget, double ridge, double* r2, int flags) {

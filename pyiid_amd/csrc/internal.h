@@ -180,6 +180,9 @@ hipError_t launch_sur_force(hipStream_t st, const AtomRec* pos, int n, const int
 hipError_t launch_gram(hipStream_t st, const float* phi, int64_t n3, const double* y,
                        const Item* chunks, int n_chunks, float* part, const Range* ranges,
                        int n_chain, double* out);
+// out[b][k] = sum_{a in chain b} e_atom[k * n + a] (FP64), off: chain offsets (n_chain + 1).
+hipError_t launch_chain_feature_energy(hipStream_t st, const float* e_atom, int64_t n,
+                                       const int64_t* off, int n_chain, double* out);
 hipError_t launch_disp_check(hipStream_t st, const AtomRec* pos, const double* pos_save,
                              const int* atom_chain, int64_t n, double lim, int* viol);
 

@@ -416,6 +416,27 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const float* __restric
   }
 }
 
+// Feature energies of the fit: out[b][k] = sum over chain b's atoms of
+// e_atom[k][a] (the per-atom outputs of the network with w3 = e_k), in FP64
+// and a fixed order.  Block (b, k); 256 threads.
+__global__ __launch_bounds__(256) void chain_feature_energy_kernel(const float* __restrict__ e_atom,
+                                                                   int64_t n,
+                                                                   const int64_t* __restrict__ off,
+                                                                   double* __restrict__ out) {
+  __shared__ double red[256];
+  const int b = blockIdx.x, k = blockIdx.y;
+  const float* row = e_atom + static_cast<size_t>(k) * n;
+  double sum = 0.0;
+  for (int64_t a = off[b] + threadIdx.x; a < off[b + 1]; a += 256) sum += static_cast<double>(row[a]);
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[static_cast<size_t>(b) * 64 + k] = red[0];
+}
+
 // Flags chains with an atom farther than sqrt(lim2) from its proposal start
 // (the neighbour list would no longer hold every pair inside the cutoff).
 __global__ void disp_check_kernel(const AtomRec* __restrict__ pos,
@@ -480,6 +501,14 @@ hipError_t launch_gram(hipStream_t st, const float* phi, int64_t n3, const doubl
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(gram_reduce_kernel, dim3(n_chain), dim3(256), 0, st, part, ranges, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_chain_feature_energy(hipStream_t st, const float* e_atom, int64_t n,
+                                       const int64_t* off, int n_chain, double* out) {
+  if (n_chain == 0) return hipSuccess;
+  hipLaunchKernelGGL(chain_feature_energy_kernel, dim3(n_chain, 64), dim3(256), 0, st, e_atom, n,
+                     off, out);
   return hipGetLastError();
 }
 

@@ -198,16 +198,27 @@ class SurrogateHMC(DeviceHMC):
 
     Metropolis still uses the exact energy (conv * Rw from the Debye path), so
     the stationary distribution is that of exact-force HMC.  The network's
-    output layer is refitted on exact forces at the chains' current states
-    every ``train_every`` proposals (pyiid_amd.surrogate)."""
+    output layer is refitted every ``train_every`` proposals on the exact
+    forces at the chains' current states and on the exact energy change since
+    the previous fit, weighted by ``energy_weight`` (A^-2; 0 fits forces
+    only; pyiid_amd.surrogate)."""
 
     def __init__(self, scat, structures, targets, potential="rw", conv=1.0, masses=None,
-                 device=0, net=None, net_seed=0, max_nbr=None):
+                 device=0, net=None, net_seed=0, max_nbr=None, energy_weight=1.0):
         from . import surrogate
         DeviceHMC.__init__(self, scat, structures, targets, potential, conv, masses, device)
         self.net = net if net is not None else surrogate.SurrogateNet(net_seed)
         self.max_nbr = surrogate.MAX_NBR if max_nbr is None else int(max_nbr)
         self.net.normalise(self.ctx, int(self.offsets[-1]), self.max_nbr)
+        self.ctx.sur_set_energy_weight(energy_weight)
+
+    def output_weights(self):
+        """The output layer in use, (B, 64) float32."""
+        return self.ctx.sur_get_output(self.B)
+
+    def feature_energies(self):
+        """Chain energies of the 64 output units at the current state, (B, 64)."""
+        return self.ctx.sur_feature_energies(self.B)
 
     def train(self, forget=0.0, ridge=1e-6):
         """One fit at the current state; returns the per-chain r2."""

@@ -246,3 +246,71 @@ def test_low_acceptance_switches_to_exact_forces():
                   min_acceptance=0.0)
     np.testing.assert_array_equal(out["exact_mode"], [0] * 20)
     hmc.close()
+
+
+def test_feature_energies_are_unit_output_chain_energies():
+    chains = _chains([30, 12], seed=2)
+    hmc, _, _, _ = _sampler(chains)
+    pos, off, chain = _state(hmc)
+    n = len(pos)
+    ef = hmc.feature_energies()
+    assert ef.shape == (hmc.B, 64)
+    d = hmc.ctx.sur_descriptors(n).astype(np.float64)
+    for k in [0, 17, 63]:
+        w3 = np.zeros((hmc.B, 64), np.float32)
+        w3[:, k] = 1.0
+        e, _ = OS.network(d, hmc.net.as_dict(), w3, chain, bf16_round=True)
+        want = np.array([e[off[b]:off[b + 1]].sum() for b in range(hmc.B)])
+        np.testing.assert_allclose(ef[:, k], want, rtol=2e-3, atol=2e-3 * np.abs(want).max())
+    hmc.close()
+
+
+def _two_state_fit(chains, weight, ridge):
+    """Train at the start and again after two exact proposals (forget 1);
+    returns the device's output layer and each state's (phi, F, e, E)."""
+    hmc, targets, e, pq = _sampler(chains, conv=50.0, energy_weight=weight)
+    pos, off, _ = _state(hmc)
+    n = len(pos)
+    samples = []
+    for step in range(2):
+        if step:
+            _, acc = hmc.run_exact(2, n_leapfrog=3, step=0.002, beta=5.0, seed=21)
+            assert acc.any()
+        hmc.train(forget=1.0, ridge=ridge)
+        phi = hmc.ctx.sur_features(n).astype(np.float64)
+        ef = hmc.feature_energies()
+        en, f = hmc.ctx.pdf_energy_forces(_lib.SLOT_PDF, pq, targets, _lib.POT_RW, 50.0, n)
+        samples.append((phi, 2.0 * f, ef, en))   # exact -dE/dq (reference convention is half)
+    w = hmc.output_weights().astype(np.float64)
+    hmc.close()
+    return w, samples, off
+
+
+def test_energy_rows_enter_the_fit():
+    """The output layer after two training states equals the oracle's fit
+    over both (each state's exact forces plus, weighted, the exact energy
+    change between them), checked through its predictions: the forces at the
+    second state and the energy change.  A heavy energy weight moves the
+    fitted energy change onto the exact one."""
+    chains = _chains([35, 20], seed=4)
+    ridge = 1e-3
+    gap = {}
+    for weight in (0.0, 1e5):
+        w_dev, samples, off = _two_state_fit(chains, weight, ridge)
+        for b in range(len(chains)):
+            cols = slice(3 * off[b], 3 * off[b + 1])
+            st = [(phi[:, cols], f[off[b]:off[b + 1]].reshape(-1), ef[b], en[b])
+                  for phi, f, ef, en in samples]
+            w_ref = OS.fit_over_states(st, 1.0, ridge, weight)
+            de = st[1][2] - st[0][2]
+            dE = st[1][3] - st[0][3]
+            f_dev, f_ref = st[1][0].T @ w_dev[b], st[1][0].T @ w_ref
+            print("\nweight %g chain %d: |dF|/|F| %.2e  dE %.4e fit %.4e ref %.4e"
+                  % (weight, b, np.linalg.norm(f_dev - f_ref) / np.linalg.norm(f_ref), dE,
+                     de @ w_dev[b], de @ w_ref))
+            assert np.linalg.norm(f_dev - f_ref) <= 2e-2 * np.linalg.norm(f_ref)
+            assert abs(de @ w_dev[b] - de @ w_ref) <= 2e-2 * max(abs(de @ w_ref), abs(dE))
+            gap[weight, b] = abs(de @ w_ref - dE)
+    for b in range(len(chains)):
+        assert gap[1e5, b] < 0.5 * gap[0.0, b]
+        assert gap[1e5, b] < 0.1 * abs(samples[1][3][b] - samples[0][3][b])

@@ -78,3 +78,58 @@ def test_ridge_fit_recovers_exact_weights():
     got, r2 = S.ridge_fit(phi, phi.T @ w, 1e-12)
     np.testing.assert_allclose(got, w, rtol=1e-6, atol=1e-8)
     assert r2 > 1 - 1e-10
+
+
+def _states(rs, n_states=3, m=120):
+    return [(rs.normal(size=(64, m)), rs.normal(size=m), rs.normal(size=64), rs.normal())
+            for _ in range(n_states)]
+
+
+def test_fit_over_states_is_the_weighted_stacked_least_squares():
+    # rows of state t carry sqrt(forget^(T-1-t)); energy row t (t >= 1) has
+    # the same decay and sqrt(weight); ridge rows sqrt(lam) I: a QR solve of
+    # that stacked system must give the normal-equation answer
+    rs = np.random.RandomState(1)
+    forget, ridge, weight = 0.5, 1e-3, 7.0
+    st = _states(rs)
+    T = len(st)
+    rows, rhs = [], []
+    for t, (phi, y, e, E) in enumerate(st):
+        s = np.sqrt(forget ** (T - 1 - t))
+        rows.append(s * phi.T)
+        rhs.append(s * y)
+        if t > 0:
+            rows.append(s * np.sqrt(weight) * (e - st[t - 1][2])[None, :])
+            rhs.append(s * np.sqrt(weight) * np.array([E - st[t - 1][3]]))
+    A = np.vstack(rows)
+    b = np.concatenate(rhs)
+    lam = ridge * np.trace(A.T @ A) / 64
+    A = np.vstack([A, np.sqrt(lam) * np.eye(64)])
+    b = np.concatenate([b, np.zeros(64)])
+    want = np.linalg.lstsq(A, b, rcond=None)[0]
+    np.testing.assert_allclose(S.fit_over_states(st, forget, ridge, weight), want, rtol=1e-9,
+                               atol=1e-12)
+    # weight 0 drops the energy rows; a single state is ridge_fit
+    np.testing.assert_allclose(S.fit_over_states(st[:1], forget, ridge, weight),
+                               S.ridge_fit(st[0][0], st[0][1], ridge)[0], rtol=1e-10)
+
+
+def test_energy_rows_recover_a_consistent_network():
+    # forces and energies of one linear head w_true: the fit recovers it, and
+    # energy rows alone (forces 0 at few columns) pin the energy direction
+    rs = np.random.RandomState(2)
+    w_true = rs.normal(size=64)
+    st = []
+    for _ in range(4):
+        phi = rs.normal(size=(64, 200))
+        e = rs.normal(size=64)
+        st.append((phi, phi.T @ w_true, e, float(e @ w_true) + 3.0))  # constant offset is free
+    np.testing.assert_allclose(S.fit_over_states(st, 1.0, 1e-12, 5.0), w_true, rtol=1e-6,
+                               atol=1e-8)
+    # energy changes inconsistent with the forces move the fit towards them
+    bent = [(p, y, e, E + 10.0 * k) for k, (p, y, e, E) in enumerate(st)]
+    w0 = S.fit_over_states(bent, 1.0, 1e-12, 0.0)
+    w1 = S.fit_over_states(bent, 1.0, 1e-12, 1e3)
+    de = bent[-1][2] - bent[-2][2]
+    dE = bent[-1][3] - bent[-2][3]
+    assert abs(de @ w1 - dE) < 0.5 * abs(de @ w0 - dE)
