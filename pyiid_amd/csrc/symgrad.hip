@@ -6,8 +6,8 @@
 //    work item is a rectangle of the lower triangle of tile pairs: an I
 //    group [g0, g1) of tiles against a J range [r0, r1) of tiles of one
 //    element, pairs (I, J) with J <= I.
-//  * A block is 8 waves of 16 consecutive Q bins (128 bins); every wave works
-//    on the same pairs.  Lane l = 16 R + p (row R of 16 lanes, place p) holds
+//  * A block is W waves of M consecutive Q bins each (16 x 8 or 12 x 12,
+//    128 or 144 bins); every wave works on the same pairs.  Lane l = 16 R + p (row R of 16 lanes, place p) holds
 //    atom I_l.  A tile pair (I, J) runs in 4 phases d of 16 steps s: lane
 //    (R, p) forms the pair (I_l, J_b) with b = 16 ((R + d) mod 4) +
 //    ((p + s) mod 16), so the 64 steps cover the 64 x 64 pairs once.  A
@@ -22,7 +22,7 @@
 //    computed once per pair and handing t over costs no instruction.  The
 //    partner's (q_i - q_j)/r^2 comes from the shared setup.  Between phases
 //    the J sums move up one row (ds_bpermute, 48 values).
-//  * The per-pair setup (FP64 geometry, 2 cos(dtheta) as hi + lo, the 8
+//  * The per-pair setup (FP64 geometry, 2 cos(dtheta) as hi + lo, the W
 //    waves' anchors sin, cos(Q0 r)) is computed once per block, one phase at a
 //    time, and shared through LDS, as in the ordered kernel.
 //  * Outputs go to slots of 64 rows x 3 x nq floats ([c][q][row], coalesced
