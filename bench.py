@@ -121,10 +121,11 @@ def cpu_baseline(pos, table, index, target_s=15.0):
     per_atom = table[index]
     n = len(pos)
     pairs = n * (n - 1) // 2
-    probe = np.array([NQ // 2], np.int32)
+    # per-bin cost from 8 strided bins (one bin would mostly time the threads' start-up)
+    probe = np.linspace(0, NQ - 1, 8).round().astype(np.int32)
     t0 = time.perf_counter()
     native.flat_f32(pos, per_atom, QBIN, q_index=probe, nthreads=threads, lib=lib)
-    dt = time.perf_counter() - t0
+    dt = (time.perf_counter() - t0) / len(probe)
     nsel = int(max(2, min(NQ, target_s / max(dt, 1e-3))))
     qi = np.unique(np.linspace(0, NQ - 1, nsel).round().astype(np.int32))
     t0 = time.perf_counter()
