@@ -283,8 +283,8 @@ int pyiid_sur_features(pyiid_ctx *ctx, float *phi, int flags);
 int pyiid_sur_feature_energies(pyiid_ctx *ctx, double *e, int flags);
 /* The output weights in use (n_chain x 64 floats, the BF16 values). */
 int pyiid_sur_get_output(pyiid_ctx *ctx, float *w3, int flags);
-/* Weight lambda (A^-2, >= 0; default 1) of the energy rows of the fit (see
- * pyiid_sur_train); 0 fits forces only. */
+/* Weight lambda (A^-2, >= 0; default 0, forces only) of the energy rows of
+ * the fit (see pyiid_sur_train). */
 int pyiid_sur_set_energy_weight(pyiid_ctx *ctx, double weight);
 /* One training step at the current positions: exact energies E_b and forces
  * F_b (Debye path, as in pyiid_hmc_setup), feature forces Phi_b and feature

@@ -155,7 +155,7 @@ struct Sur {
   std::vector<double> r2;   // fit quality of the last training sample per chain
   // energy-difference rows: weight (A^-2) and the previous training state's
   // feature energies (B x 64) and exact energies (B)
-  double energy_weight = 1.0;
+  double energy_weight = 0.0;
   bool eprev_set = false;
   std::vector<double> eprev, Eprev;
 };

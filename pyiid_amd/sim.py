@@ -200,11 +200,11 @@ class SurrogateHMC(DeviceHMC):
     the stationary distribution is that of exact-force HMC.  The network's
     output layer is refitted every ``train_every`` proposals on the exact
     forces at the chains' current states and on the exact energy change since
-    the previous fit, weighted by ``energy_weight`` (A^-2; 0 fits forces
-    only; pyiid_amd.surrogate)."""
+    the previous fit, weighted by ``energy_weight`` (A^-2; the default 0
+    fits forces only; pyiid_amd.surrogate)."""
 
     def __init__(self, scat, structures, targets, potential="rw", conv=1.0, masses=None,
-                 device=0, net=None, net_seed=0, max_nbr=None, energy_weight=1.0):
+                 device=0, net=None, net_seed=0, max_nbr=None, energy_weight=0.0):
         from . import surrogate
         DeviceHMC.__init__(self, scat, structures, targets, potential, conv, masses, device)
         self.net = net if net is not None else surrogate.SurrogateNet(net_seed)

@@ -11,7 +11,7 @@ Network: 16 Gaussian radial basis functions (centres 2.0 .. 6.0 A, cutoff
 accumulation on the matrix cores.  The hidden layers are fixed seeded random
 features; the output layer w3 is fitted per chain, online, by ridge regression
 of the feature forces on exact forces that the Debye kernels produce at the
-chains' current states during sampling, together with one row per fit that
+chains' current states during sampling, optionally with one row per fit that
 matches the network's energy change since the previous fit to the exact
 change in conv * Rw (forces and energies are linear in w3, so each fit is a
 64 x 64 solve).  When the fit is poor the sampler falls back to exact forces
