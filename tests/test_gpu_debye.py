@@ -268,6 +268,7 @@ def test_symmetric_every_group_size(ctx):
     """The symmetric (unordered-pair) F + gradient kernel at each grouping of
     64-atom tiles per work item (4, 8, 16, 32), fixed instead of autotuned,
     against the oracle:
+    - 2 and 3 atoms: one tile of one or three pairs;
     - 50 and 64 atoms: one tile, the diagonal steps only (a partial and a
       full tile);
     - 65 atoms: a second tile of one atom;
@@ -278,13 +279,16 @@ def test_symmetric_every_group_size(ctx):
       element boundary and every slot takes f_I(Q) f_J(Q).
     Groupings sum in different orders, so the 1,100-atom gradients must
     differ in the last bits between 4 and 32 (equal outputs would mean the
-    setting did not reach the launch)."""
-    cases = ((50, 250, 0.1, False), (64, 250, 0.1, False), (65, 250, 0.1, False),
-             (203, 330, 0.0756865, False), (1100, 200, 0.1, False), (400, 300, 0.08, True))
+    setting did not reach the launch).  The 12-bin, 12-wave shape runs every
+    case at 8 tiles per item."""
+    cases = ((2, 100, 0.1, False), (3, 100, 0.1, False), (50, 250, 0.1, False),
+             (64, 250, 0.1, False), (65, 250, 0.1, False), (203, 330, 0.0756865, False),
+             (1100, 200, 0.1, False), (400, 300, 0.08, True))
     out = {}
     try:
-        for k in (4, 8, 16, 32):
+        for k, m in ((4, 16), (8, 16), (16, 16), (32, 16), (8, 12)):
             ctx.set_grad_tiles_per_item(k)
+            ctx.set_grad_sym_bins_per_wave(m)
             for n, nq, qbin, alloy in cases:
                 pos = structures.jitter(structures.fcc_nearest(n, a=3.9), 0.05,
                                         seed=n).astype(np.float32)
@@ -292,19 +296,20 @@ def test_symmetric_every_group_size(ctx):
                 per_atom = load(ctx, pos, numbers, qbin, nq)
                 fq_out = np.empty(nq, np.float32)
                 got = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_out, f64=True)
-                assert ctx.grad_shape() == (True, 16, k)
+                assert ctx.grad_shape() == (True, m, k)
                 rows = np.arange(0, n, 7) if n > 500 else np.arange(n)
                 want = oracle.grad_fq(pos, per_atom, qbin, atom_index=rows)
                 eg = rel(got[rows], want)
                 ef = rel(fq_out, oracle.fq(pos, per_atom, qbin))
-                print("k=%d n=%d nq=%d: grad %.2e F %.2e" % (k, n, nq, eg, ef), flush=True)
+                print("k=%d m=%d n=%d nq=%d: grad %.2e F %.2e" % (k, m, n, nq, eg, ef), flush=True)
                 assert eg < TOL and ef < TOL
-                out[(k, n)] = got
-        assert not np.array_equal(out[(4, 1100)], out[(32, 1100)])
+                out[(k, m, n)] = got
+        assert not np.array_equal(out[(4, 16, 1100)], out[(32, 16, 1100)])
         with pytest.raises(ValueError, match="tiles per item"):
             ctx.set_grad_tiles_per_item(12)
     finally:
         ctx.set_grad_tiles_per_item(0)
+        ctx.set_grad_sym_bins_per_wave(0)
 
 
 def test_grad_kernel_choice(ctx):
