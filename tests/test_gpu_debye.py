@@ -276,14 +276,16 @@ def test_symmetric_every_group_size(ctx):
     - 1,100 atoms at 200 bins: 18 tiles, so groups of 4 and 8 give items whose
       J-row slots take the adds of several I tiles;
     - a Pt/Ni alloy of 400 atoms at 300 bins: tiles and J ranges end at the
-      element boundary and every slot takes f_I(Q) f_J(Q).
+      element boundary and every slot takes f_I(Q) f_J(Q);
+    - an Au/Pt/Ni particle of 300 atoms (uneven thirds, 70 Ni) at 260 bins:
+      two element boundaries, one of them inside the second tile.
     Groupings sum in different orders, so the 1,100-atom gradients must
     differ in the last bits between 4 and 32 (equal outputs would mean the
     setting did not reach the launch).  The 12-bin, 12-wave shape runs every
     case at 8 tiles per item."""
     cases = ((2, 100, 0.1, False), (3, 100, 0.1, False), (50, 250, 0.1, False),
              (64, 250, 0.1, False), (65, 250, 0.1, False), (203, 330, 0.0756865, False),
-             (1100, 200, 0.1, False), (400, 300, 0.08, True))
+             (1100, 200, 0.1, False), (400, 300, 0.08, True), (300, 260, 0.09, "three"))
     out = {}
     try:
         for k, m in ((4, 16), (8, 16), (16, 16), (32, 16), (8, 12)):
@@ -292,7 +294,13 @@ def test_symmetric_every_group_size(ctx):
             for n, nq, qbin, alloy in cases:
                 pos = structures.jitter(structures.fcc_nearest(n, a=3.9), 0.05,
                                         seed=n).astype(np.float32)
-                numbers = structures.random_alloy(n, 78, 28, seed=n) if alloy else np.full(n, 79)
+                if alloy == "three":
+                    numbers = np.array([79] * 120 + [78] * 110 + [28] * 70)[
+                        np.random.RandomState(n).permutation(n)]
+                elif alloy:
+                    numbers = structures.random_alloy(n, 78, 28, seed=n)
+                else:
+                    numbers = np.full(n, 79)
                 per_atom = load(ctx, pos, numbers, qbin, nq)
                 fq_out = np.empty(nq, np.float32)
                 got = ctx.grad_fq(_lib.SLOT_FQ, qbin, n, nq, fq_out=fq_out, f64=True)
