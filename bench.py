@@ -369,13 +369,17 @@ def _first_at_or_below(rw, target):
     return int(hit[0]) if hit.size else None
 
 
-def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10, energy_weight=None):
+def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10, energy_weight=None,
+                   n_leap=2, eps=3.0, sur_n_leap=None, sur_eps=None):
     """Wall-seconds to a target Rw: exact-force HMC against the learned
     integrator on 64 chains of 5,000-atom Au particles refined against the
     G(r) of independently rattled copies.  ASE masses (Au 196.97 amu),
-    energy 100 Rw, beta 1000, 10 velocity-Verlet steps of 0.56 per proposal
-    (both samplers accept about 99% of exact-force proposals at this step).
-    Per sampler seed: the exact run's plateau is its mean Rw over the last
+    energy 100 Rw, beta 1000, n_leap velocity-Verlet steps of eps per
+    proposal for both samplers (sur_n_leap / sur_eps for the learned
+    sampler when given).  The default 2 steps of 3.0 is the exact sampler's
+    fastest trajectory to the target among those it accepts at least 90% of
+    the time (a sweep over 1-10 steps of 0.56-5.6; larger steps reach the
+    target sooner at lower acceptance).  Per sampler seed: the exact run's plateau is its mean Rw over the last
     10 of its n_exact proposals, the target that plateau + 0.005, and each
     sampler's time to target the HIP-event time at which the chains' mean
     Rw first reaches it.  The learned integrator trains every train_every
@@ -387,7 +391,9 @@ def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10, energy
     from pyiid_amd import workloads
     from pyiid_amd.elasticscatter import ElasticScatter
     from pyiid_amd.sim import DeviceHMC, SurrogateHMC
-    conv, beta, eps, n_leap = 100.0, 1000.0, 0.56, 10
+    conv, beta = 100.0, 1000.0
+    sur_n_leap = n_leap if sur_n_leap is None else sur_n_leap
+    sur_eps = eps if sur_eps is None else sur_eps
     starts, targs, numbers = workloads.build("large", 0)
     scat = ElasticScatter(dict(workloads.EXP), seed=0)
     gt = workloads.target_pdfs(scat, targs, numbers)
@@ -404,7 +410,7 @@ def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10, energy
         k_e = _first_at_or_below(rw_e, target)
         kw = {} if energy_weight is None else {"energy_weight": energy_weight}
         sur = SurrogateHMC(scat, structs, gt, conv=conv, net_seed=seed, **kw)
-        out = sur.run(n_sur, n_leapfrog=n_leap, step=eps, beta=beta, seed=seed,
+        out = sur.run(n_sur, n_leapfrog=sur_n_leap, step=sur_eps, beta=beta, seed=seed,
                       train_every=train_every, forget=0.5, ridge=1e-6, min_r2=0.5,
                       min_acceptance=0.3)
         sur.close()
@@ -437,7 +443,8 @@ def hmc_comparison(n_exact=40, n_sur=80, seeds=(0, 1, 2), train_every=10, energy
     return {
         "workload": "64 Au chains x 5000 atoms (nearest FCC sites, a=4.08 A), start sigma=0.05 A, "
                     "targets from sigma=0.1 A rattled copies; Q 0-25 step 0.1, r 0-40 step 0.01",
-        "leapfrog_steps": n_leap, "step": eps, "beta": beta, "conv": conv,
+        "leapfrog_steps": n_leap, "step": eps, "learned_leapfrog_steps": sur_n_leap,
+        "learned_step": sur_eps, "beta": beta, "conv": conv,
         "masses": "ASE (Au 196.97 amu)", "seeds": list(seeds), "runs": runs,
         "targets_reached_by_learned": len(sp), "speedup_to_target_mean": (sum(sp) / len(sp)) if sp else None,
     }
